@@ -1,0 +1,168 @@
+/*
+ * felbm_mi355x.h -- C ABI of the MI355X engine for the felbm Lee-Liu
+ * free-energy multiphase D3Q19 step.
+ *
+ * The engine is the drop-in replacement for felbm_gpu::MultiPhaseGPU
+ * (include/felbm_gpu/multiphase_gpu.cuh:23-783 of the reference), the GPU
+ * peer of the CPU lbm::EngineMultiPhase.  Each declaration below names the
+ * reference member it replaces.  All calls return FELBM_OK (0) or a negative
+ * error code; felbm_last_error() describes the last failure of the calling
+ * thread.  Nothing here exits the process (the reference's GPU_CHECK calls
+ * exit(EXIT_FAILURE), gpu_common.cuh:14-22).
+ *
+ * One handle drives one GPU and holds one or more independent lattice
+ * instances ("slots").  felbm_step advances every slot in the same launches,
+ * which is both the plain engine (one slot) and the batched form (many).
+ * A handle is not thread-safe: drive it from one host thread.
+ *
+ * Layouts at the boundary are the reference's: distributions are double,
+ * population k of site i at [k*n + i] (Vn = 19*n); sites are the fluid voxels
+ * numbered in x-fastest lexicographic order; masks are uint8 [nz][ny][nx]
+ * with 1 = solid.
+ */
+#ifndef FELBM_MI355X_H
+#define FELBM_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FELBM_OK           0
+#define FELBM_EINVAL      (-1)
+#define FELBM_ENOMEM      (-2)
+#define FELBM_EHIP        (-3)
+#define FELBM_ESTATE      (-4)
+#define FELBM_EUNSUPPORTED (-5)
+#define FELBM_ENONFINITE  (-6)
+
+typedef struct felbm_engine felbm_engine;
+
+/* Geometry of one instance (replaces lbm::SubDomain, multiphase_gpu.cuh:75-79). */
+typedef struct {
+  int nx, ny, nz;
+  int periodic[3];           /* 1 = periodic axis */
+  int use_halfway_bb;        /* Settings::use_halfway_bb */
+  const uint8_t *solid;      /* [nz][ny][nx], 1 = solid voxel */
+} felbm_geometry;
+
+/* Physics (ParametersMultiPhase + Settings fields read at
+ * multiphase_gpu.cuh:331-343; derived values as compare_cpu_gpu.cu:111-118). */
+typedef struct {
+  double rho0, rho1;                 /* phase_density(0), phase_density(1) */
+  double kappa, beta;
+  double boundary_coefficient;
+  double tau0, tau1;                 /* relaxation_time(0), relaxation_time(1) */
+  double mobility;
+  double forcing_factor;
+  double acceleration[3];            /* Settings::acceleration */
+  double forcing[3];                 /* Settings::forcing */
+  double mrt_lambda;                 /* Settings::mrt_lambda */
+} felbm_params;
+
+/* Operator switches (felbm_gpu_main.cu:210-222).  The six fast-path switches
+ * select algebraically equivalent paths in the reference; this engine always
+ * runs its fused form and accepts them for configuration compatibility. */
+typedef struct {
+  int use_mrt;
+  int correct_op_mass;
+  int stream_matrix_free, grad_matrix_free, fused, fuse_collision;
+  int mrt_fast_transform, stream_inplace;
+} felbm_switches;
+
+/* ---- lifecycle -------------------------------------------------------------
+ * precision: 8 = FP64 distributions (reference double build),
+ *            4 = FP32 distributions with FP64 reductions (-DFELBM_SINGLE). */
+int  felbm_create(felbm_engine **out, int device, int precision);
+int  felbm_destroy(felbm_engine *e);                         /* MultiPhaseGPU::free, :759 */
+/* MultiPhaseGPU::init (multiphase_gpu.cuh:75-351); returns the slot id. */
+int  felbm_add_instance(felbm_engine *e, const felbm_geometry *geom,
+                        const felbm_params *prm, const felbm_switches *sw, int *slot);
+int  felbm_remove_instance(felbm_engine *e, int slot);
+int  felbm_num_sites(felbm_engine *e, int slot, int64_t *n);  /* SubDomain::size_sites */
+int  felbm_state_size(felbm_engine *e, int slot, int64_t *vn);/* state_size, :457 */
+int  felbm_num_instances(felbm_engine *e, int *count);
+
+/* ---- state ---------------------------------------------------------------- */
+int  felbm_upload_state(felbm_engine *e, int slot, const double *h, const double *g);   /* :354 */
+int  felbm_download_state(felbm_engine *e, int slot, double *h, double *g);            /* :449 */
+/* rest equilibrium h_k = w_k c, g_k = w_k p0 from a per-site c (n values) */
+int  felbm_init_rest(felbm_engine *e, int slot, const double *c_site, double p0);
+
+/* ---- stepping -------------------------------------------------------------
+ * step() (multiphase_gpu.cuh:522-631) followed by apply_mass_correction()
+ * (:689-700) when correct_op_mass is on, nsteps times, for every slot.
+ * stats_last != 0 accumulates the monitoring sums during the final step. */
+int  felbm_step(felbm_engine *e, int nsteps, int stats_last);
+int  felbm_set_forcing_factor(felbm_engine *e, int slot, double ff);   /* P.forcing_factor */
+int  felbm_synchronize(felbm_engine *e);
+/* capture `nsteps` (even) steps in a HIP graph and replay it `replays` times */
+int  felbm_step_graph(felbm_engine *e, int nsteps, int replays);
+
+/* ---- mass correction (record_target_mass / get/set / current_mass) -------- */
+int  felbm_record_target_mass(felbm_engine *e, int slot);               /* :685 */
+int  felbm_get_target_mass(felbm_engine *e, int slot, double *m);       /* :458 */
+int  felbm_set_target_mass(felbm_engine *e, int slot, double m);        /* :459 */
+int  felbm_current_mass(felbm_engine *e, int slot, double *m);          /* :687 */
+
+/* ---- monitoring: flow_stats (:645-683) ------------------------------------
+ * out = { sum ux, sum uy, sum uz, sum u^2, sum c, max u^2 } over the fields
+ * of the last step run with stats_last != 0; if none, over a fresh field
+ * pass of the current state. */
+int  felbm_flow_stats(felbm_engine *e, int slot, double out[6]);
+int  felbm_flow_stats_launch(felbm_engine *e);                 /* async copy-back */
+int  felbm_flow_stats_ready(felbm_engine *e, int *ready);
+int  felbm_flow_stats_collect(felbm_engine *e, int slot, double out[6]);
+
+/* ---- fields: download / download_raw (:704-757) ---------------------------
+ * out = rho, c, ux, uy, uz, p, each n doubles, recomputed from the current
+ * state exactly as the reference's download does. */
+int  felbm_download_fields(felbm_engine *e, int slot, double *out);
+
+/* ---- open boundaries: set_open_bnd (:369-400) + the DevParams open-BC
+ * block (device_engine.cuh:45-58, filled at felbm_gpu_main.cu:340-364). */
+typedef struct {
+  int use_in_vel, use_in_prs, use_in_fluid, use_out_prs, use_out_fluid;
+  double u_in[3];
+  double p_in, p_out;
+  double c_out_fixed, rho_out_fixed;
+  double inlet_c_a;
+  int inlet_mode;          /* 0 single, 1 alternate, 2 split, 3 stripes */
+  int split_axis;
+  double inlet_period, inlet_duty, inlet_ramp, split_pos;
+} felbm_open_bnd;
+int  felbm_set_open_bnd(felbm_engine *e, int slot, const felbm_open_bnd *ob,
+                        const int32_t *inlet, int n_inlet,
+                        const int32_t *outlet, int n_outlet,
+                        const double *inlet_coord,
+                        const int32_t *buf_verts, const int32_t *buf_refs, int n_buf,
+                        int copy_to_buffers);
+
+/* ---- verification helpers -------------------------------------------------
+ * Reference-format tables (multiphase_gpu.cuh:93-208) regenerated on the GPU
+ * from the engine's own neighbour logic, for bitwise comparison. Any pointer
+ * may be NULL. (Q-1)*n entries except avgnext and src (Q*n). */
+int  felbm_export_tables(felbm_engine *e, int slot, int32_t *cdm, int32_t *cdp,
+                         int32_t *bcase, int32_t *bc1, int32_t *bc2, int32_t *avgnext,
+                         int32_t *lcase, int32_t *lc1, int32_t *lc2, int32_t *src);
+/* the 19x19 MRT moment matrix and its inverse used by the engine */
+int  felbm_mrt_matrices(double *M, double *Minv);
+/* last kernel timings of one step, in ms: moments, c-stencil, mu-stencil, collide */
+int  felbm_kernel_times(felbm_engine *e, int nsteps, float *ms4);
+
+const char *felbm_last_error(void);
+const char *felbm_version(void);
+
+/* ---- geometry generators (DomainInitializer_* stand-ins) ------------------ */
+int     felbm_geom_cylinders(int nx, int ny, int nz, int count, double rmin, double rmax,
+                             double gap, uint64_t seed, int max_tries, uint8_t *solid);
+int     felbm_geom_spheres(int nx, int ny, int nz, double rmin, double rmax, double porosity,
+                           uint64_t seed, uint8_t *solid);
+int64_t felbm_keep_largest_cluster(int nx, int ny, int nz, int px, int py, int pz,
+                                   uint8_t *solid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
