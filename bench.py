@@ -1,0 +1,130 @@
+"""Benchmark of the Lee-Liu D3Q19 step on one MI355X.
+
+One step = one full time step of the hot path (moments, c stencil, mu
+stencil, fused MRT collision with push streaming, FP64 mass correction) over
+the 300^3 random-sphere pack at porosity 0.37 (about 1e7 fluid sites), FP32
+distributions with FP64 reductions -- the configuration the headline MLUPS is
+quoted on.  MLUPS = fluid sites x steps / wall seconds, as the reference's
+driver reports it (src/felbm_gpu_main.cu:785-788).
+
+Prints one JSON line (rank 0).  The CPU baseline beside it is the FP64 OpenMP
+oracle (a restatement of the reference step; the reference's own CPU engine
+is not available) timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic values per site per launch (FP32 -> x4 bytes); see DESIGN.md
+ALG_VALUES = {"moments": 43, "cfield": 8, "mufield": 2, "collide": 61}
+B_ALG_PER_LU = 452  # bytes per lattice update, FP32 (113 values)
+
+
+def _profile_traffic(workload):
+    """HBM bytes per collide launch from the committed rocprof counter summary."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("collide_hbm_bytes_per_launch")
+    return None
+
+
+def cpu_baseline(workload, w, steps):
+    from oracle.pyoracle import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = Oracle(w["solid"], w["params"], use_mrt=w["switches"]["use_mrt"], fuse_collision=True,
+               stream_inplace=False, mass_mode=1 if w["switches"]["correct_op_mass"] else 0)
+    o.init_rest(w["c0"], w["p0"])
+    o.record_target_mass()
+    o.run(1)
+    t0 = time.perf_counter()
+    o.run(steps)
+    dt = time.perf_counter() - t0
+    n = o.n
+    del o
+    return {"value": n * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": "%s FP64 OpenMP oracle, %d steps x %d sites, %.1f s" % (workload, steps, n, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="porous_300")
+    ap.add_argument("--cpu-steps", type=int, default=int(os.environ.get("FELBM_CPU_STEPS", "6")))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU figures are not part of this build "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from felbm_gpu_amd import MultiPhaseGPU
+    from felbm_gpu_amd import workloads as W
+
+    t0 = time.time()
+    w = W.workload(args.workload)
+    print("[bench] workload %s built in %.1fs" % (args.workload, time.time() - t0), file=sys.stderr, flush=True)
+    eng = MultiPhaseGPU(precision=w["precision"]).init(w["solid"], w["params"], w["switches"])
+    eng.init_rest(w["c0"], w["p0"])
+    eng.record_target_mass()
+    n = eng.n
+    print("[bench] sites %d, precision %s" % (n, w["precision"]), file=sys.stderr, flush=True)
+
+    eng.step(args.warmup)
+    eng.batch.synchronize()
+    t0 = time.perf_counter()
+    eng.step(args.steps)
+    eng.batch.synchronize()
+    dt = time.perf_counter() - t0
+    mlups = n * args.steps / dt / 1e6
+    ms = eng.batch.kernel_times(max(5, min(50, args.steps)))
+    names = ["moments", "cfield", "mufield", "collide"]
+    bytes_per_val = 4 if w["precision"] == "f32" else 8
+    kern = {}
+    for k, t in zip(names, ms):
+        alg = ALG_VALUES[k] * bytes_per_val * n
+        kern[k] = {"ms": t, "alg_GBs": alg / (t * 1e-3) / 1e9 if t > 0 else None}
+    coll = kern["collide"]
+    roof = {"bound": "hbm", "achieved": coll["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": coll["alg_GBs"] / HBM_PEAK_GBS, "traffic": _profile_traffic(args.workload),
+            "kernel": "k_collide (fused MRT + push streaming)",
+            "alg_bytes_per_launch": ALG_VALUES["collide"] * bytes_per_val * n}
+    mass_drift = abs(eng.current_mass() - eng.get_target_mass())
+    print("[bench] %.1f MLUPS, kernels %s" % (mlups, json.dumps(kern)), file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline (%d steps)" % args.cpu_steps, file=sys.stderr, flush=True)
+        eng.free()
+        cpu = cpu_baseline(args.workload, w, args.cpu_steps)
+
+    out = {
+        "metric": "MLUPS (300^3 fused MRT) + batched-sweep MLUPS and p50/p99 solve latency",
+        "value": mlups, "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if w["precision"] == "f32" else "f64",
+        "data": "synthetic (seeded random-sphere pack, slab initial condition)",
+        "config": {"workload": args.workload, "sites": n, "collision": "MRT" if w["switches"]["use_mrt"] else "BGK",
+                   "correct_op_mass": bool(w["switches"]["correct_op_mass"]),
+                   "hbm_frac_total": mlups * 1e6 * B_ALG_PER_LU / (HBM_PEAK_GBS * 1e9)},
+        "kernels_ms": {k: v["ms"] for k, v in kern.items()},
+        "mass_drift": mass_drift,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,88 @@
+// capi.cpp -- the extern "C" boundary (include/felbm_mi355x.h).
+#include <hip/hip_runtime.h>
+#include <string>
+#include <new>
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace fm {
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+}
+
+struct felbm_engine { fm::EngineBase *impl; int precision; };
+
+#define ENG(e) do { if (!(e) || !(e)->impl) { fm::set_error("null engine handle"); return FELBM_EINVAL; } } while (0)
+
+extern "C" {
+
+const char *felbm_last_error(void) { return fm::g_err.c_str(); }
+const char *felbm_version(void) { return "felbm-mi355x 0.1 (gfx950)"; }
+
+int felbm_create(felbm_engine **out, int device, int precision)
+{
+  if (!out) { fm::set_error("null output pointer"); return FELBM_EINVAL; }
+  *out = nullptr;
+  if (precision != 8 && precision != 4) { fm::set_error("precision must be 8 (FP64) or 4 (FP32)"); return FELBM_EINVAL; }
+  int err = FELBM_OK;
+  fm::EngineBase *impl = precision == 8 ? fm::make_engine_f64(device, &err) : fm::make_engine_f32(device, &err);
+  if (!impl) return err ? err : FELBM_EHIP;
+  felbm_engine *e = new (std::nothrow) felbm_engine{impl, precision};
+  if (!e) { delete impl; fm::set_error("out of host memory"); return FELBM_ENOMEM; }
+  *out = e;
+  return FELBM_OK;
+}
+
+int felbm_destroy(felbm_engine *e)
+{
+  if (!e) return FELBM_OK;
+  delete e->impl; delete e;
+  return FELBM_OK;
+}
+
+int felbm_add_instance(felbm_engine *e, const felbm_geometry *g, const felbm_params *p, const felbm_switches *s, int *slot)
+{ ENG(e); return e->impl->add_instance(g, p, s, slot); }
+int felbm_remove_instance(felbm_engine *e, int slot) { ENG(e); return e->impl->remove_instance(slot); }
+int felbm_num_sites(felbm_engine *e, int slot, int64_t *n) { ENG(e); if (!n) return FELBM_EINVAL; return e->impl->num_sites(slot, n); }
+int felbm_state_size(felbm_engine *e, int slot, int64_t *vn)
+{ ENG(e); if (!vn) return FELBM_EINVAL; int64_t n; int r = e->impl->num_sites(slot, &n); if (!r) *vn = 19 * n; return r; }
+int felbm_num_instances(felbm_engine *e, int *count) { ENG(e); if (!count) return FELBM_EINVAL; return e->impl->num_instances(count); }
+int felbm_upload_state(felbm_engine *e, int slot, const double *h, const double *g) { ENG(e); return e->impl->upload_state(slot, h, g); }
+int felbm_download_state(felbm_engine *e, int slot, double *h, double *g)
+{ ENG(e); if (!h || !g) { fm::set_error("null output"); return FELBM_EINVAL; } return e->impl->download_state(slot, h, g); }
+int felbm_init_rest(felbm_engine *e, int slot, const double *c, double p0)
+{ ENG(e); if (!c) { fm::set_error("null c"); return FELBM_EINVAL; } return e->impl->init_rest(slot, c, p0); }
+int felbm_step(felbm_engine *e, int nsteps, int stats_last) { ENG(e); return e->impl->step(nsteps, stats_last); }
+int felbm_step_graph(felbm_engine *e, int nsteps, int replays) { ENG(e); return e->impl->step_graph(nsteps, replays); }
+int felbm_set_forcing_factor(felbm_engine *e, int slot, double ff) { ENG(e); return e->impl->set_forcing_factor(slot, ff); }
+int felbm_synchronize(felbm_engine *e) { ENG(e); return e->impl->synchronize(); }
+int felbm_record_target_mass(felbm_engine *e, int slot) { ENG(e); return e->impl->record_target_mass(slot); }
+int felbm_get_target_mass(felbm_engine *e, int slot, double *m) { ENG(e); if (!m) return FELBM_EINVAL; return e->impl->get_target_mass(slot, m); }
+int felbm_set_target_mass(felbm_engine *e, int slot, double m) { ENG(e); return e->impl->set_target_mass(slot, m); }
+int felbm_current_mass(felbm_engine *e, int slot, double *m) { ENG(e); if (!m) return FELBM_EINVAL; return e->impl->current_mass(slot, m); }
+int felbm_flow_stats(felbm_engine *e, int slot, double out[6]) { ENG(e); if (!out) return FELBM_EINVAL; return e->impl->flow_stats(slot, out); }
+int felbm_flow_stats_launch(felbm_engine *e) { ENG(e); return e->impl->flow_stats_launch(); }
+int felbm_flow_stats_ready(felbm_engine *e, int *ready) { ENG(e); if (!ready) return FELBM_EINVAL; return e->impl->flow_stats_ready(ready); }
+int felbm_flow_stats_collect(felbm_engine *e, int slot, double out[6]) { ENG(e); if (!out) return FELBM_EINVAL; return e->impl->flow_stats_collect(slot, out); }
+int felbm_download_fields(felbm_engine *e, int slot, double *out) { ENG(e); if (!out) return FELBM_EINVAL; return e->impl->download_fields(slot, out); }
+int felbm_set_open_bnd(felbm_engine *e, int slot, const felbm_open_bnd *ob, const int32_t *inlet, int n_inlet,
+                       const int32_t *outlet, int n_outlet, const double *icoord,
+                       const int32_t *bv, const int32_t *br, int n_buf, int copy_to_buffers)
+{ ENG(e); return e->impl->set_open_bnd(slot, ob, inlet, n_inlet, outlet, n_outlet, icoord, bv, br, n_buf, copy_to_buffers); }
+int felbm_export_tables(felbm_engine *e, int slot, int32_t *cdm, int32_t *cdp, int32_t *bcase, int32_t *bc1,
+                        int32_t *bc2, int32_t *avgnext, int32_t *lcase, int32_t *lc1, int32_t *lc2, int32_t *src)
+{
+  ENG(e);
+  int32_t *t[10] = { cdm, cdp, bcase, bc1, bc2, avgnext, lcase, lc1, lc2, src };
+  return e->impl->export_tables(slot, t);
+}
+int felbm_kernel_times(felbm_engine *e, int nsteps, float *ms4) { ENG(e); if (!ms4) return FELBM_EINVAL; return e->impl->kernel_times(nsteps, ms4); }
+
+int felbm_mrt_matrices(double *M, double *Minv)
+{
+  if (!M || !Minv) return FELBM_EINVAL;
+  for (int a = 0; a < 19; ++a) for (int b = 0; b < 19; ++b) { M[a*19 + b] = fm::MRTTAB.M[a][b]; Minv[a*19 + b] = fm::MRTTAB.MI[a][b]; }
+  return FELBM_OK;
+}
+
+}  // extern "C"

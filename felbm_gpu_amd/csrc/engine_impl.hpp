@@ -1,0 +1,606 @@
+// engine_impl.hpp -- host side of the engine, templated on the distribution
+// precision.  Included by engine_f64.hip and engine_f32.hip, which are built
+// with different floating-point contraction settings.
+//
+// Replaces felbm_gpu::MultiPhaseGPU (multiphase_gpu.cuh:23-783): owns the
+// device state of every instance, runs the step, the mass corrector, the
+// monitoring reductions and the state/field transfers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace fm {
+
+#define FM_HIP(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) {                 \
+    set_error(std::string("HIP error ") + hipGetErrorName(e_) + " at " + __FILE__ + ":" +   \
+              std::to_string(__LINE__) + " (" #call ")"); return FELBM_EHIP; } } while (0)
+#define FM_LAUNCH() FM_HIP(hipGetLastError())
+
+inline int ceil_log2(int v) { int b = 0; while ((1 << b) < v) ++b; return b; }
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+template <typename real>
+struct Slot {
+  bool used = false;
+  int n = 0, ns = 0, nblocks = 0;
+  int Nx = 0, Ny = 0, Nz = 0, wpr = 0;
+  int per[3] = {1, 1, 1};
+  int hbb = 1, mrt = 0, mass_on = 0;
+  int par0 = 0;
+  double M0 = 0.0;
+  double params[32];
+  felbm_params prm{};
+  char *base = nullptr;           // single device allocation
+  size_t bytes = 0;
+  Inst<real> d{};                 // descriptor (device pointers filled)
+  int2 *dmap = nullptr;           // single-instance block map
+  int dmap_len = 0;
+  int stats_valid = 0;
+};
+
+template <typename real>
+class Engine : public EngineBase {
+public:
+  explicit Engine(int device) : dev_(device) {}
+  ~Engine() override { release(); }
+
+  int init(int *err) {
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) { set_error("no HIP device available"); *err = FELBM_EHIP; return *err; }
+    if (dev_ < 0 || dev_ >= nd) { set_error("device index out of range"); *err = FELBM_EINVAL; return *err; }
+    if (hipSetDevice(dev_) != hipSuccess) { set_error("hipSetDevice failed"); *err = FELBM_EHIP; return *err; }
+    if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) { set_error("stream create failed"); *err = FELBM_EHIP; return *err; }
+    for (int q = 0; q < 10; ++q) if (hipEventCreate(&ev_[q]) != hipSuccess) { set_error("event create failed"); *err = FELBM_EHIP; return *err; }
+    if (hipHostMalloc((void **)&pinned_stats_, sizeof(unsigned long long)*12*NSLOT*kMaxAsyncSlots, hipHostMallocDefault) != hipSuccess) { *err = FELBM_ENOMEM; return *err; }
+    *err = FELBM_OK;
+    return FELBM_OK;
+  }
+
+  // ---- lifecycle ---------------------------------------------------------------
+  int add_instance(const felbm_geometry *g, const felbm_params *p, const felbm_switches *s, int *slot) override {
+    if (!g || !p || !s || !slot || !g->solid) { set_error("null argument"); return FELBM_EINVAL; }
+    if (g->nx <= 0 || g->ny <= 0 || g->nz <= 0) { set_error("box dimensions must be positive"); return FELBM_EINVAL; }
+    int bx = ceil_log2(g->nx), by = ceil_log2(g->ny), bz = ceil_log2(g->nz);
+    if (bx + by + bz > 32) { set_error("box too large for 32-bit packed coordinates"); return FELBM_EUNSUPPORTED; }
+    FM_HIP(hipSetDevice(dev_));
+    Slot<real> S;
+    S.Nx = g->nx; S.Ny = g->ny; S.Nz = g->nz;
+    for (int a = 0; a < 3; ++a) S.per[a] = g->periodic[a] ? 1 : 0;
+    S.hbb = g->use_halfway_bb ? 1 : 0;
+    S.mrt = s->use_mrt ? 1 : 0;
+    S.mass_on = s->correct_op_mass ? 1 : 0;
+    S.prm = *p;
+    // lattice words + packed coordinates
+    const int wpr = (g->nx + 31) / 32;
+    const int64_t rows = (int64_t)g->ny * g->nz;
+    std::vector<uint64_t> words((size_t)(rows * wpr));
+    std::vector<uint32_t> coords;
+    int64_t cnt = 0;
+    for (int z = 0; z < g->nz; ++z)
+      for (int y = 0; y < g->ny; ++y) {
+        const uint8_t *row = g->solid + ((int64_t)z * g->ny + y) * g->nx;
+        for (int w = 0; w < wpr; ++w) {
+          uint32_t mask = 0; uint64_t start = (uint64_t)cnt;
+          for (int b = 0; b < 32; ++b) {
+            int x = w * 32 + b;
+            if (x >= g->nx) break;
+            if (!row[x]) {
+              mask |= 1u << b;
+              coords.push_back((uint32_t)x | ((uint32_t)y << bx) | ((uint32_t)z << (bx + by)));
+              ++cnt;
+            }
+          }
+          words[(size_t)((((int64_t)z * g->ny) + y) * wpr + w)] = (start << 32) | mask;
+        }
+      }
+    if (cnt == 0) { set_error("geometry has no fluid sites"); return FELBM_EINVAL; }
+    if (cnt >= (int64_t)1 << 31) { set_error("more than 2^31 sites"); return FELBM_EUNSUPPORTED; }
+    S.n = (int)cnt; S.ns = (int)align_up((size_t)cnt, 64); S.wpr = wpr;
+    S.nblocks = (S.n + 255) / 256;
+    // one allocation: h, g0, g1 (19 ns), 9 fields + 4 export (ns), words, coords, acc
+    const size_t RB = sizeof(real), ns = (size_t)S.ns;
+    size_t off = 0;
+    auto take = [&](size_t b) { size_t o = off; off = align_up(off + b, 256); return o; };
+    size_t o_h = take(19 * ns * RB), o_g0 = take(19 * ns * RB), o_g1 = take(19 * ns * RB);
+    size_t o_f = take(13 * ns * RB);
+    size_t o_w = take(words.size() * 8), o_c = take(coords.size() * 4);
+    size_t o_a = take(sizeof(unsigned long long) * ACC_WORDS);
+    size_t o_m = take(sizeof(int2) * (size_t)(align_up((size_t)S.nblocks, 8)));
+    S.bytes = off;
+    if (hipMalloc((void **)&S.base, off) != hipSuccess) { set_error("device allocation failed"); return FELBM_ENOMEM; }
+    FM_HIP(hipMemsetAsync(S.base, 0, off, stream_));
+    FM_HIP(hipMemcpyAsync(S.base + o_w, words.data(), words.size() * 8, hipMemcpyHostToDevice, stream_));
+    FM_HIP(hipMemcpyAsync(S.base + o_c, coords.data(), coords.size() * 4, hipMemcpyHostToDevice, stream_));
+    Inst<real> &d = S.d;
+    d.n = S.n; d.ns = S.ns; d.Nx = S.Nx; d.Ny = S.Ny; d.Nz = S.Nz; d.wpr = wpr;
+    d.sx = bx; d.sxy = bx + by; d.mx = (1u << bx) - 1u; d.my = (1u << by) - 1u;
+    d.px = S.per[0]; d.py = S.per[1]; d.pz = S.per[2];
+    d.hbb = S.hbb; d.mrt = S.mrt; d.mass_on = S.mass_on;
+    d.words = (const uint64_t *)(S.base + o_w); d.coords = (const uint32_t *)(S.base + o_c);
+    d.h = (real *)(S.base + o_h); d.g0 = (real *)(S.base + o_g0); d.g1 = (real *)(S.base + o_g1);
+    real *f = (real *)(S.base + o_f);
+    d.craw = f; d.p0 = f + ns; d.jx = f + 2 * ns; d.jy = f + 3 * ns; d.jz = f + 4 * ns;
+    d.c = f + 5 * ns; d.mu = f + 6 * ns; d.ps = f + 7 * ns; d.lapmu = f + 8 * ns;
+    export_off_ = 9;   // exports at f + 9*ns .. 12*ns
+    d.acc = (unsigned long long *)(S.base + o_a);
+    set_params(S);
+    // admission parity: g0 holds the state at the current step counter
+    S.par0 = (int)(T_ & 1u); d.par0 = S.par0;
+    // single-instance block map
+    S.dmap = (int2 *)(S.base + o_m);
+    S.dmap_len = (int)align_up((size_t)S.nblocks, 8);
+    int sid = -1;
+    for (size_t q = 0; q < slots_.size(); ++q) if (!slots_[q].used) { sid = (int)q; break; }
+    if (sid < 0) { sid = (int)slots_.size(); slots_.push_back(Slot<real>()); }
+    S.used = true;
+    slots_[sid] = S;
+    dirty_ = true;
+    if (int e = rebuild()) return e;
+    *slot = sid;
+    return FELBM_OK;
+  }
+
+  int remove_instance(int slot) override {
+    if (int e = check(slot)) return e;
+    FM_HIP(hipStreamSynchronize(stream_));
+    FM_HIP(hipFree(slots_[slot].base));
+    slots_[slot] = Slot<real>();
+    dirty_ = true;
+    return rebuild();
+  }
+
+  int num_sites(int slot, int64_t *n) override { if (int e = check(slot)) return e; *n = slots_[slot].n; return FELBM_OK; }
+  int num_instances(int *count) override {
+    int c = 0; for (auto &s : slots_) c += s.used; *count = c; return FELBM_OK;
+  }
+
+  // ---- state -------------------------------------------------------------------
+  int upload_state(int slot, const double *h, const double *g) override {
+    if (int e = check(slot)) return e;
+    if (!h || !g) { set_error("null state"); return FELBM_EINVAL; }
+    Slot<real> &S = slots_[slot];
+    const size_t n = (size_t)S.n, ns = (size_t)S.ns;
+    std::vector<real> t(19 * ns, (real)0);
+    unsigned cur = cur_of(S);
+    for (int pass = 0; pass < 2; ++pass) {
+      const double *src = pass ? g : h;
+      for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) t[k * ns + i] = (real)src[k * n + i];
+      real *dst = pass ? (cur ? S.d.g1 : S.d.g0) : S.d.h;
+      FM_HIP(hipMemcpyAsync(dst, t.data(), t.size() * sizeof(real), hipMemcpyHostToDevice, stream_));
+      FM_HIP(hipStreamSynchronize(stream_));
+    }
+    FM_HIP(hipMemsetAsync(S.d.acc + ACC_FLAGS, 0, sizeof(unsigned long long), stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    S.stats_valid = 0;
+    return FELBM_OK;
+  }
+
+  int init_rest(int slot, const double *c, double p0) override {
+    if (int e = check(slot)) return e;
+    Slot<real> &S = slots_[slot];
+    const size_t n = (size_t)S.n;
+    std::vector<double> h(19 * n), g(19 * n);
+    for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) { h[k * n + i] = c[i] * WD[k]; g[k * n + i] = WD[k] * p0; }
+    return upload_state(slot, h.data(), g.data());
+  }
+
+  int download_state(int slot, double *h, double *g) override {
+    if (int e = check(slot)) return e;
+    Slot<real> &S = slots_[slot];
+    if (int e = flush()) return e;
+    const size_t n = (size_t)S.n, ns = (size_t)S.ns;
+    unsigned cur = cur_of(S);
+    real *stage = cur ? S.d.g0 : S.d.g1;      // the dead g buffer between steps
+    bool apply = false; double lambda = 0.0;
+    if (int e = owed_lambda(S, apply, lambda)) return e;
+    int nb = (S.n + 255) / 256;
+    hipLaunchKernelGGL(k_inject_copy<real>, dim3(nb), dim3(256), 0, stream_, S.d, apply ? 1 : 0, lambda, stage);
+    FM_LAUNCH();
+    std::vector<real> t(19 * ns);
+    FM_HIP(hipMemcpyAsync(t.data(), stage, t.size() * sizeof(real), hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) h[k * n + i] = (double)t[k * ns + i];
+    FM_HIP(hipMemcpyAsync(t.data(), cur ? S.d.g1 : S.d.g0, t.size() * sizeof(real), hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) g[k * n + i] = (double)t[k * ns + i];
+    return FELBM_OK;
+  }
+
+  // ---- stepping ------------------------------------------------------------------
+  int step(int nsteps, int stats_last) override {
+    if (nsteps < 0) { set_error("negative step count"); return FELBM_EINVAL; }
+    if (int e = flush()) return e;
+    if (nmap_ == 0) return FELBM_OK;
+    for (int t = 0; t < nsteps; ++t) {
+      bool st = stats_last && t == nsteps - 1;
+      if (int e = launch_step(st)) return e;
+    }
+    if (nsteps > 0) for (auto &s : slots_) if (s.used) s.stats_valid = stats_last ? 1 : 0;
+    return FELBM_OK;
+  }
+
+  int step_graph(int nsteps, int replays) override {
+    if (nsteps <= 0 || (nsteps & 1) || replays < 0) { set_error("graph steps must be a positive even count"); return FELBM_EINVAL; }
+    if (int e = flush()) return e;
+    if (nmap_ == 0) return FELBM_OK;
+    if (!graph_exec_ || graph_steps_ != nsteps || graph_ver_ != map_ver_ || (graph_T_ & 1u) != (T_ & 1u)) {
+      if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+      unsigned T0 = T_;
+      FM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      for (int t = 0; t < nsteps; ++t) {
+        int e = launch_step(false);
+        if (e) { hipGraph_t gg; (void)hipStreamEndCapture(stream_, &gg); T_ = T0; return e; }
+      }
+      hipGraph_t graph;
+      FM_HIP(hipStreamEndCapture(stream_, &graph));
+      FM_HIP(hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      T_ = T0;
+      graph_steps_ = nsteps; graph_ver_ = map_ver_; graph_T_ = T0;
+    }
+    for (int r = 0; r < replays; ++r) { FM_HIP(hipGraphLaunch(graph_exec_, stream_)); T_ += (unsigned)nsteps; }
+    for (auto &s : slots_) if (s.used) s.stats_valid = 0;
+    return FELBM_OK;
+  }
+
+  int set_forcing_factor(int slot, double ff) override {
+    if (int e = check(slot)) return e;
+    slots_[slot].prm.forcing_factor = ff;
+    slots_[slot].d.ff = (real)ff;
+    dirty_desc_ = true;
+    return FELBM_OK;
+  }
+  int synchronize() override { FM_HIP(hipStreamSynchronize(stream_)); return FELBM_OK; }
+
+  // ---- mass correction ------------------------------------------------------------
+  int record_target_mass(int slot) override {
+    double m; if (int e = current_mass(slot, &m)) return e;
+    slots_[slot].M0 = m; slots_[slot].d.M0 = m; dirty_desc_ = true;
+    return FELBM_OK;
+  }
+  int get_target_mass(int slot, double *m) override { if (int e = check(slot)) return e; *m = slots_[slot].M0; return FELBM_OK; }
+  int set_target_mass(int slot, double m) override {
+    if (int e = check(slot)) return e;
+    slots_[slot].M0 = m; slots_[slot].d.M0 = m; dirty_desc_ = true; return FELBM_OK;
+  }
+  int current_mass(int slot, double *m) override {
+    if (int e = check(slot)) return e;
+    if (int e = flush()) return e;
+    Slot<real> &S = slots_[slot];
+    bool apply = false; double lambda = 0.0;
+    if (int e = owed_lambda(S, apply, lambda)) return e;
+    double M, W; uint64_t bad;
+    if (int e = mass_sums(S, apply, lambda, M, W, bad)) return e;
+    if (bad) { set_error("non-finite or out-of-range order parameter in the state"); return FELBM_ENONFINITE; }
+    *m = M;
+    return FELBM_OK;
+  }
+
+  // ---- monitoring -------------------------------------------------------------------
+  int flow_stats(int slot, double *out) override {
+    if (int e = check(slot)) return e;
+    Slot<real> &S = slots_[slot];
+    if (!S.stats_valid) return stats_from_fields(S, out);
+    std::vector<unsigned long long> a(12 * NSLOT);
+    FM_HIP(hipMemcpyAsync(a.data(), S.d.acc + ACC_STATS, a.size() * 8, hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    return reduce_stats(a.data(), out);
+  }
+  int flow_stats_launch() override {
+    int q = 0;
+    for (auto &s : slots_) {
+      if (s.used) FM_HIP(hipMemcpyAsync(pinned_stats_ + (size_t)q * 12 * NSLOT, s.d.acc + ACC_STATS,
+                                        12 * NSLOT * 8, hipMemcpyDeviceToHost, stream_));
+      if (++q >= kMaxAsyncSlots) break;
+    }
+    FM_HIP(hipEventRecord(ev_[9], stream_));
+    stats_launched_ = true;
+    return FELBM_OK;
+  }
+  int flow_stats_ready(int *ready) override {
+    if (!stats_launched_) { *ready = 0; return FELBM_OK; }
+    hipError_t e = hipEventQuery(ev_[9]);
+    *ready = (e == hipSuccess) ? 1 : 0;
+    if (e != hipSuccess && e != hipErrorNotReady) { set_error("event query failed"); return FELBM_EHIP; }
+    return FELBM_OK;
+  }
+  int flow_stats_collect(int slot, double *out) override {
+    if (int e = check(slot)) return e;
+    if (!stats_launched_) { set_error("flow_stats_launch was not called"); return FELBM_ESTATE; }
+    if (slot >= kMaxAsyncSlots) { set_error("async stats cover the first 256 slots"); return FELBM_EUNSUPPORTED; }
+    FM_HIP(hipEventSynchronize(ev_[9]));
+    return reduce_stats(pinned_stats_ + (size_t)slot * 12 * NSLOT, out);
+  }
+
+  // ---- fields ----------------------------------------------------------------------
+  int download_fields(int slot, double *out) override {
+    if (int e = check(slot)) return e;
+    Slot<real> &S = slots_[slot];
+    if (int e = field_pass(S)) return e;
+    const size_t n = (size_t)S.n, ns = (size_t)S.ns;
+    real *exp = S.d.craw + export_off_ * ns;
+    std::vector<real> t(4 * ns), cc(ns), pp(ns);
+    FM_HIP(hipMemcpyAsync(t.data(), exp, t.size() * sizeof(real), hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipMemcpyAsync(cc.data(), S.d.c, ns * sizeof(real), hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipMemcpyAsync(pp.data(), S.d.ps, ns * sizeof(real), hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < n; ++i) {
+      out[i] = (double)t[i]; out[n + i] = (double)cc[i];
+      out[2 * n + i] = (double)t[ns + i]; out[3 * n + i] = (double)t[2 * ns + i]; out[4 * n + i] = (double)t[3 * ns + i];
+      out[5 * n + i] = (double)pp[i];
+    }
+    return FELBM_OK;
+  }
+
+  int set_open_bnd(int, const felbm_open_bnd *, const int32_t *, int, const int32_t *, int, const double *,
+                   const int32_t *, const int32_t *, int, int) override {
+    set_error("open boundaries are not implemented in this engine build");
+    return FELBM_EUNSUPPORTED;
+  }
+
+  int export_tables(int slot, int32_t **tabs) override {
+    if (int e = check(slot)) return e;
+    if (int e = flush()) return e;
+    Slot<real> &S = slots_[slot];
+    const size_t n = (size_t)S.n, Q1n = 18 * n, Vn = 19 * n;
+    size_t sz[10] = { Q1n, Q1n, Q1n, Q1n, Q1n, Vn, Q1n, Q1n, Q1n, Vn };
+    size_t tot = 0; for (int q = 0; q < 10; ++q) tot += sz[q];
+    int *buf = nullptr;
+    FM_HIP(hipMalloc((void **)&buf, tot * 4));
+    int *p[10]; size_t o = 0;
+    for (int q = 0; q < 10; ++q) { p[q] = buf + o; o += sz[q]; }
+    hipLaunchKernelGGL(k_export_tables<real>, dim3((S.n + 255) / 256), dim3(256), 0, stream_, S.d,
+                       p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9]);
+    hipError_t le = hipGetLastError();
+    std::vector<int> h(tot);
+    if (le == hipSuccess) le = hipMemcpyAsync(h.data(), buf, tot * 4, hipMemcpyDeviceToHost, stream_);
+    if (le == hipSuccess) le = hipStreamSynchronize(stream_);
+    (void)hipFree(buf);
+    if (le != hipSuccess) { set_error(std::string("export_tables: ") + hipGetErrorName(le)); return FELBM_EHIP; }
+    o = 0;
+    for (int q = 0; q < 10; ++q) { if (tabs[q]) std::memcpy(tabs[q], h.data() + o, sz[q] * 4); o += sz[q]; }
+    return FELBM_OK;
+  }
+
+  int kernel_times(int nsteps, float *ms4) override {
+    if (int e = flush()) return e;
+    if (nmap_ == 0 || nsteps <= 0) { for (int q = 0; q < 4; ++q) ms4[q] = 0; return FELBM_OK; }
+    double acc[4] = {0, 0, 0, 0};
+    for (int t = 0; t < nsteps; ++t) {
+      if (int e = launch_step(false, true)) return e;
+      FM_HIP(hipEventSynchronize(ev_[4]));
+      for (int q = 0; q < 4; ++q) { float ms = 0; FM_HIP(hipEventElapsedTime(&ms, ev_[q], ev_[q + 1])); acc[q] += ms; }
+    }
+    for (int q = 0; q < 4; ++q) ms4[q] = (float)(acc[q] / nsteps);
+    for (auto &s : slots_) if (s.used) s.stats_valid = 0;
+    return FELBM_OK;
+  }
+
+private:
+  static constexpr int kMaxAsyncSlots = 256;
+  int dev_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev_[10] = {};
+  std::vector<Slot<real>> slots_;
+  std::vector<int> slot_desc_;      // slot -> descriptor index
+  Inst<real> *d_desc_ = nullptr; int desc_cap_ = 0;
+  int2 *d_map_ = nullptr, *d_map_mrt_ = nullptr, *d_map_bgk_ = nullptr;
+  int map_cap_ = 0;
+  int nmap_ = 0, nmap_mrt_ = 0, nmap_bgk_ = 0;
+  bool dirty_ = false, dirty_desc_ = false;
+  unsigned T_ = 0;
+  unsigned map_ver_ = 0;
+  int export_off_ = 9;
+  unsigned long long *pinned_stats_ = nullptr;
+  bool stats_launched_ = false;
+  hipGraphExec_t graph_exec_ = nullptr;
+  int graph_steps_ = 0; unsigned graph_ver_ = ~0u, graph_T_ = 0;
+
+  int check(int slot) {
+    if (slot < 0 || slot >= (int)slots_.size() || !slots_[slot].used) { set_error("invalid slot"); return FELBM_EINVAL; }
+    return FELBM_OK;
+  }
+  unsigned cur_of(const Slot<real> &S) const { return (T_ ^ (unsigned)S.par0) & 1u; }
+
+  void set_params(Slot<real> &S) {
+    const felbm_params &p = S.prm;
+    Inst<real> &d = S.d;
+    d.rho0 = (real)p.rho0; d.rho1 = (real)p.rho1; d.drho = d.rho0 - d.rho1;
+    d.kappa = (real)p.kappa; d.four_beta = (real)(4.0 * p.beta); d.bnd_coeff = (real)p.boundary_coefficient;
+    d.tau0 = (real)p.tau0; d.tau1 = (real)p.tau1; d.mobility = (real)p.mobility; d.mrt_lambda = (real)p.mrt_lambda;
+    d.gx = (real)p.acceleration[0]; d.gy = (real)p.acceleration[1]; d.gz = (real)p.acceleration[2];
+    d.fx = (real)p.forcing[0]; d.fy = (real)p.forcing[1]; d.fz = (real)p.forcing[2];
+    d.ff = (real)p.forcing_factor;
+    d.M0 = S.M0;
+  }
+
+  void release() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    for (auto &s : slots_) if (s.used && s.base) (void)hipFree(s.base);
+    slots_.clear();
+    if (d_desc_) (void)hipFree(d_desc_);
+    if (d_map_) (void)hipFree(d_map_);
+    if (pinned_stats_) (void)hipHostFree(pinned_stats_);
+    for (auto &e : ev_) if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+    stream_ = nullptr;
+  }
+
+  // rebuild descriptor array and block maps after membership changes
+  int rebuild() {
+    slot_desc_.assign(slots_.size(), -1);
+    std::vector<Inst<real>> descs;
+    std::vector<int2> all, mrt, bgk;
+    for (size_t q = 0; q < slots_.size(); ++q) {
+      if (!slots_[q].used) continue;
+      int di = (int)descs.size();
+      slot_desc_[q] = di;
+      descs.push_back(slots_[q].d);
+      for (int b = 0; b < slots_[q].nblocks; ++b) {
+        int2 e = make_int2(di, b * 256);
+        all.push_back(e);
+        (slots_[q].mrt ? mrt : bgk).push_back(e);
+      }
+    }
+    auto pad = [](std::vector<int2> &v) { while (v.size() % 8) v.push_back(make_int2(-1, 0)); };
+    pad(all); pad(mrt); pad(bgk);
+    nmap_ = (int)all.size(); nmap_mrt_ = (int)mrt.size(); nmap_bgk_ = (int)bgk.size();
+    int need = nmap_ + nmap_mrt_ + nmap_bgk_;
+    if ((int)descs.size() > desc_cap_) {
+      if (d_desc_) FM_HIP(hipFree(d_desc_));
+      desc_cap_ = std::max((int)descs.size(), 2 * desc_cap_);
+      FM_HIP(hipMalloc((void **)&d_desc_, sizeof(Inst<real>) * desc_cap_));
+    }
+    if (need > map_cap_) {
+      if (d_map_) FM_HIP(hipFree(d_map_));
+      map_cap_ = std::max(need, 2 * map_cap_);
+      FM_HIP(hipMalloc((void **)&d_map_, sizeof(int2) * map_cap_));
+    }
+    d_map_mrt_ = d_map_ + nmap_; d_map_bgk_ = d_map_mrt_ + nmap_mrt_;
+    if (!descs.empty()) FM_HIP(hipMemcpyAsync(d_desc_, descs.data(), sizeof(Inst<real>) * descs.size(), hipMemcpyHostToDevice, stream_));
+    if (nmap_) FM_HIP(hipMemcpyAsync(d_map_, all.data(), sizeof(int2) * nmap_, hipMemcpyHostToDevice, stream_));
+    if (nmap_mrt_) FM_HIP(hipMemcpyAsync(d_map_mrt_, mrt.data(), sizeof(int2) * nmap_mrt_, hipMemcpyHostToDevice, stream_));
+    if (nmap_bgk_) FM_HIP(hipMemcpyAsync(d_map_bgk_, bgk.data(), sizeof(int2) * nmap_bgk_, hipMemcpyHostToDevice, stream_));
+    // single-instance maps (field downloads) carry the new descriptor indices
+    std::vector<std::vector<int2>> single(slots_.size());
+    for (size_t q = 0; q < slots_.size(); ++q) {
+      if (!slots_[q].used) continue;
+      single[q].assign((size_t)slots_[q].dmap_len, make_int2(-1, 0));
+      for (int b = 0; b < slots_[q].nblocks; ++b) single[q][b] = make_int2(slot_desc_[q], b * 256);
+      FM_HIP(hipMemcpyAsync(slots_[q].dmap, single[q].data(), single[q].size() * sizeof(int2), hipMemcpyHostToDevice, stream_));
+    }
+    FM_HIP(hipStreamSynchronize(stream_));
+    dirty_ = false; dirty_desc_ = false;
+    ++map_ver_;
+    return FELBM_OK;
+  }
+
+  int flush() {
+    if (dirty_) return rebuild();
+    if (dirty_desc_) {
+      std::vector<Inst<real>> descs;
+      for (auto &s : slots_) if (s.used) descs.push_back(s.d);
+      if (!descs.empty()) FM_HIP(hipMemcpyAsync(d_desc_, descs.data(), sizeof(Inst<real>) * descs.size(), hipMemcpyHostToDevice, stream_));
+      FM_HIP(hipStreamSynchronize(stream_));
+      dirty_desc_ = false;
+    }
+    return FELBM_OK;
+  }
+
+  int zero_stats() {
+    for (auto &s : slots_) if (s.used) {
+      hipLaunchKernelGGL(k_zero_words<real>, dim3((12 * NSLOT + 255) / 256), dim3(256), 0, stream_, s.d.acc + ACC_STATS, 12 * NSLOT);
+      FM_LAUNCH();
+    }
+    return FELBM_OK;
+  }
+
+  int launch_step(bool stats, bool timed = false) {
+    if (stats) if (int e = zero_stats()) return e;
+    dim3 blk(256);
+    if (timed) FM_HIP(hipEventRecord(ev_[0], stream_));
+    hipLaunchKernelGGL(k_moments<real>, dim3(nmap_), blk, 0, stream_, d_desc_, d_map_, T_); FM_LAUNCH();
+    if (timed) FM_HIP(hipEventRecord(ev_[1], stream_));
+    hipLaunchKernelGGL(k_cfield<real>, dim3(nmap_), blk, 0, stream_, d_desc_, d_map_, T_, (real *)nullptr, -1); FM_LAUNCH();
+    if (timed) FM_HIP(hipEventRecord(ev_[2], stream_));
+    hipLaunchKernelGGL(k_mufield<real>, dim3(nmap_), blk, 0, stream_, d_desc_, d_map_, T_); FM_LAUNCH();
+    if (timed) FM_HIP(hipEventRecord(ev_[3], stream_));
+    if (nmap_mrt_) { hipLaunchKernelGGL((k_collide<real, true>), dim3(nmap_mrt_), blk, 0, stream_, d_desc_, d_map_mrt_, T_, stats ? 1 : 0); FM_LAUNCH(); }
+    if (nmap_bgk_) { hipLaunchKernelGGL((k_collide<real, false>), dim3(nmap_bgk_), blk, 0, stream_, d_desc_, d_map_bgk_, T_, stats ? 1 : 0); FM_LAUNCH(); }
+    if (timed) FM_HIP(hipEventRecord(ev_[4], stream_));
+    ++T_;
+    return FELBM_OK;
+  }
+
+  // lambda owed by the state (a correction is pending after any step)
+  int owed_lambda(Slot<real> &S, bool &apply, double &lambda) {
+    apply = false; lambda = 0.0;
+    if (!S.mass_on) return FELBM_OK;
+    unsigned long long pend = 0;
+    FM_HIP(hipMemcpyAsync(&pend, S.d.acc + ACC_FLAGS, 8, hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    if (!pend) return FELBM_OK;
+    double M, W; uint64_t bad;
+    if (int e = mass_sums(S, false, 0.0, M, W, bad)) return e;
+    double drift = M - S.M0;
+    apply = (W > 0.0 && drift != 0.0);
+    lambda = apply ? drift / W : 0.0;
+    return FELBM_OK;
+  }
+
+  int mass_sums(Slot<real> &S, bool apply, double lambda, double &M, double &W, uint64_t &bad) {
+    FM_HIP(hipMemsetAsync(S.d.acc + ACC_MISC, 0, 5 * NSLOT * 8, stream_));
+    hipLaunchKernelGGL(k_mass_single<real>, dim3((S.n + 255) / 256), dim3(256), 0, stream_, S.d, cur_of(S), apply ? 1 : 0, lambda);
+    FM_LAUNCH();
+    std::vector<unsigned long long> a(5 * NSLOT);
+    FM_HIP(hipMemcpyAsync(a.data(), S.d.acc + ACC_MISC, a.size() * 8, hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    unsigned long long s[5] = {0, 0, 0, 0, 0};
+    for (int q = 0; q < 5; ++q) for (int l = 0; l < NSLOT; ++l) s[q] += a[q * NSLOT + l];
+    M = fx_value((long long)s[0], s[1]); W = fx_value((long long)s[2], s[3]); bad = s[4];
+    return FELBM_OK;
+  }
+
+  int reduce_stats(const unsigned long long *a, double *out) {
+    unsigned long long s[12] = {0};
+    for (int q = 0; q < 11; ++q) for (int l = 0; l < NSLOT; ++l) s[q] += a[q * NSLOT + l];
+    for (int l = 0; l < NSLOT; ++l) s[11] = std::max(s[11], a[11 * NSLOT + l]);
+    for (int q = 0; q < 5; ++q) out[q] = fx_value((long long)s[2 * q], s[2 * q + 1]);
+    double mx; std::memcpy(&mx, &s[11], 8); out[5] = mx;
+    if (s[10]) { set_error("non-finite or out-of-range value in the monitored fields"); return FELBM_ENONFINITE; }
+    return FELBM_OK;
+  }
+
+  // field pass on one instance: moments + c stencil with export (download_raw)
+  int field_pass(Slot<real> &S) {
+    if (int e = flush()) return e;
+    const int di = slot_desc_[&S - slots_.data()];
+    const size_t ns = (size_t)S.ns;
+    dim3 blk(256), grid(S.dmap_len);
+    hipLaunchKernelGGL(k_moments<real>, grid, blk, 0, stream_, d_desc_, S.dmap, T_); FM_LAUNCH();
+    hipLaunchKernelGGL(k_cfield<real>, grid, blk, 0, stream_, d_desc_, S.dmap, T_, S.d.craw + export_off_ * ns, di); FM_LAUNCH();
+    // the step's own moments pass must find this parity's partials at zero
+    unsigned cur = cur_of(S);
+    FM_HIP(hipMemsetAsync(S.d.acc + ACC_MASS + cur * 5 * NSLOT, 0, 5 * NSLOT * 8, stream_));
+    return FELBM_OK;
+  }
+
+  int stats_from_fields(Slot<real> &S, double *out) {
+    std::vector<double> f((size_t)6 * S.n);
+    if (int e = download_fields((int)(&S - slots_.data()), f.data())) return e;
+    // exact fixed-point sums on the host, same definition as the kernels
+    long long hi[5] = {0}; unsigned long long lo[5] = {0}, bad = 0; double mx = 0;
+    const size_t n = (size_t)S.n;
+    for (size_t i = 0; i < n; ++i) {
+      double vx = f[2 * n + i], vy = f[3 * n + i], vz = f[4 * n + i];
+      double v2 = vx * vx + vy * vy + vz * vz;
+      double t[5] = { vx, vy, vz, v2, f[n + i] };
+      for (int q = 0; q < 5; ++q) {
+        if (!(std::fabs(t[q]) < 8.0)) { ++bad; continue; }
+        long long qq = (long long)std::rint(t[q] * 0x1p60);
+        hi[q] += qq >> 32; lo[q] += (unsigned long long)(qq & 0xffffffffLL);
+      }
+      if (v2 > mx) mx = v2;
+    }
+    for (int q = 0; q < 5; ++q) out[q] = fx_value(hi[q], lo[q]);
+    out[5] = mx;
+    if (bad) { set_error("non-finite or out-of-range value in the monitored fields"); return FELBM_ENONFINITE; }
+    return FELBM_OK;
+  }
+};
+
+template <typename real>
+EngineBase *make_engine(int device, int *err) {
+  Engine<real> *e = new Engine<real>(device);
+  if (e->init(err) != FELBM_OK) { delete e; return nullptr; }
+  return e;
+}
+
+}  // namespace fm

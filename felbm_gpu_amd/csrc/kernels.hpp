@@ -68,7 +68,7 @@ constexpr MrtTab make_mrt()
   }
   return t;
 }
-constexpr MrtTab MRT = make_mrt();
+constexpr MrtTab MRTTAB = make_mrt();
 
 // Per-instance descriptor (the batched DevParams, device_engine.cuh:31-59).
 template <typename real>
@@ -624,12 +624,12 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
         real yv = cg[nn];
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
-          if (MRT.M[m][nn] != 0.0) {
+          if (MRTTAB.M[m][nn] != 0.0) {
             const bool odd = (m == 4 || m == 6 || m == 8 || m >= 16);
             const bool cons = (m == 3 || m == 5 || m == 7);
             if (!cons) {
               real sm = odd ? s1r : s0r;
-              xx[m] += sm*(real)MRT.M[m][nn]*yv;
+              xx[m] += sm*(real)MRTTAB.M[m][nn]*yv;
             }
           }
         }
@@ -640,7 +640,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
         #pragma unroll
         for (int nn = 0; nn < Q; ++nn) {
           const bool cons = (nn == 0 || nn == 3 || nn == 5 || nn == 7);
-          if (MRT.MI[m][nn] != 0.0 && !cons) acc += (real)MRT.MI[m][nn]*xx[nn];
+          if (MRTTAB.MI[m][nn] != 0.0 && !cons) acc += (real)MRTTAB.MI[m][nn]*xx[nn];
         }
         real gnew = gk[m] + acc + fgs[m];
         int dst = n1[m];

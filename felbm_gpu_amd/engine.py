@@ -1,0 +1,227 @@
+"""Python mirror of ``felbm_gpu::MultiPhaseGPU`` over the C ABI.
+
+Names and argument meanings follow ``include/felbm_gpu/multiphase_gpu.cuh``
+of the reference (init, upload_state, step, apply_mass_correction,
+record_target_mass, current_mass, flow_stats*, download, download_state,
+free), so a driver or test written against the reference reads the same.
+One ``MultiPhaseGPU`` is one slot of an engine handle; ``Batch`` exposes the
+handle with many slots (the batched form).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+D3Q19_W = np.array([1.0 / 3.0] + [1.0 / 18.0] * 6 + [1.0 / 36.0] * 12)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _geom(solid, periodic=(1, 1, 1), use_halfway_bb=True):
+    solid = np.ascontiguousarray(solid, dtype=np.uint8)
+    nz, ny, nx = solid.shape
+    g = _lib.Geometry()
+    g.nx, g.ny, g.nz = nx, ny, nz
+    for a in range(3):
+        g.periodic[a] = int(bool(periodic[a]))
+    g.use_halfway_bb = int(bool(use_halfway_bb))
+    g.solid = solid.ctypes.data_as(C.POINTER(C.c_uint8))
+    return g, solid
+
+
+def _params(p):
+    q = _lib.Params()
+    for k in ("rho0", "rho1", "kappa", "beta", "boundary_coefficient", "tau0", "tau1", "mobility",
+              "forcing_factor", "mrt_lambda"):
+        setattr(q, k, float(p[k]))
+    for a in range(3):
+        q.acceleration[a] = float(p["acceleration"][a])
+        q.forcing[a] = float(p["forcing"][a])
+    return q
+
+
+def _switches(s):
+    w = _lib.Switches()
+    for k in ("use_mrt", "correct_op_mass", "stream_matrix_free", "grad_matrix_free", "fused",
+              "fuse_collision", "mrt_fast_transform", "stream_inplace"):
+        setattr(w, k, int(bool(s.get(k, 1 if k not in ("use_mrt", "correct_op_mass") else 0))))
+    return w
+
+
+class Batch:
+    """An engine handle on one GPU holding any number of lattice instances."""
+
+    def __init__(self, device=0, precision="f64"):
+        self.lib = _lib.load()
+        self.precision = {"f64": 8, "f32": 4, 8: 8, 4: 4}[precision]
+        h = C.c_void_p()
+        check(self.lib.felbm_create(C.byref(h), int(device), self.precision))
+        self.h = h
+        self._keep = {}
+
+    def add(self, solid, params, switches, periodic=(1, 1, 1), use_halfway_bb=True):
+        g, arr = _geom(solid, periodic, use_halfway_bb)
+        slot = C.c_int()
+        check(self.lib.felbm_add_instance(self.h, C.byref(g), C.byref(_params(params)),
+                                          C.byref(_switches(switches)), C.byref(slot)))
+        return slot.value
+
+    def remove(self, slot):
+        check(self.lib.felbm_remove_instance(self.h, int(slot)))
+
+    def n_sites(self, slot):
+        n = C.c_int64()
+        check(self.lib.felbm_num_sites(self.h, int(slot), C.byref(n)))
+        return n.value
+
+    def step(self, nsteps=1, stats_last=False):
+        check(self.lib.felbm_step(self.h, int(nsteps), int(bool(stats_last))))
+
+    def step_graph(self, nsteps, replays):
+        check(self.lib.felbm_step_graph(self.h, int(nsteps), int(replays)))
+
+    def synchronize(self):
+        check(self.lib.felbm_synchronize(self.h))
+
+    def kernel_times(self, nsteps):
+        ms = (C.c_float * 4)()
+        check(self.lib.felbm_kernel_times(self.h, int(nsteps), ms))
+        return list(ms)
+
+    def free(self):
+        if self.h:
+            self.lib.felbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class MultiPhaseGPU:
+    """One lattice on one GPU: the drop-in for the reference's MultiPhaseGPU."""
+
+    def __init__(self, precision="f64", device=0, batch=None):
+        self.batch = batch if batch is not None else Batch(device, precision)
+        self.lib = self.batch.lib
+        self.slot = None
+        self.n = 0
+        self.Vn = 0
+        self.correct_mass = False
+
+    # multiphase_gpu.cuh:75-351
+    def init(self, solid, params, switches, periodic=(1, 1, 1), use_halfway_bb=True):
+        self.slot = self.batch.add(solid, params, switches, periodic, use_halfway_bb)
+        self.n = self.batch.n_sites(self.slot)
+        self.Vn = 19 * self.n
+        self.correct_mass = bool(switches.get("correct_op_mass", False))
+        return self
+
+    @property
+    def h(self):
+        return self.batch.h
+
+    # :354-359
+    def upload_state(self, h, g):
+        h = np.ascontiguousarray(h, dtype=np.float64).reshape(-1)
+        g = np.ascontiguousarray(g, dtype=np.float64).reshape(-1)
+        if h.size != self.Vn or g.size != self.Vn:
+            raise ValueError("state arrays must hold 19*n values")
+        check(self.lib.felbm_upload_state(self.h, self.slot, _dptr(h), _dptr(g)))
+
+    def init_rest(self, c_site, p0=1.0):
+        c = np.ascontiguousarray(c_site, dtype=np.float64)
+        if c.size != self.n:
+            raise ValueError("c must hold n values")
+        check(self.lib.felbm_init_rest(self.h, self.slot, _dptr(c), float(p0)))
+
+    # :449-455
+    def download_state(self):
+        h = np.empty(self.Vn)
+        g = np.empty(self.Vn)
+        check(self.lib.felbm_download_state(self.h, self.slot, _dptr(h), _dptr(g)))
+        return h, g
+
+    def state_size(self):
+        return self.Vn
+
+    # :522 + :689 (the correction runs inside when correct_op_mass is on)
+    def step(self, nsteps=1, stats_last=False):
+        self.batch.step(nsteps, stats_last)
+
+    def apply_mass_correction(self):
+        """Kept for interface parity: the correction is applied by step()."""
+        return 0.0
+
+    def set_forcing_factor(self, ff):
+        check(self.lib.felbm_set_forcing_factor(self.h, self.slot, float(ff)))
+
+    def record_target_mass(self):
+        check(self.lib.felbm_record_target_mass(self.h, self.slot))
+
+    def get_target_mass(self):
+        m = C.c_double()
+        check(self.lib.felbm_get_target_mass(self.h, self.slot, C.byref(m)))
+        return m.value
+
+    def set_target_mass(self, m):
+        check(self.lib.felbm_set_target_mass(self.h, self.slot, float(m)))
+
+    def current_mass(self):
+        m = C.c_double()
+        check(self.lib.felbm_current_mass(self.h, self.slot, C.byref(m)))
+        return m.value
+
+    # :645-683
+    def flow_stats(self):
+        out = np.empty(6)
+        check(self.lib.felbm_flow_stats(self.h, self.slot, _dptr(out)))
+        return out
+
+    def flow_stats_launch(self):
+        check(self.lib.felbm_flow_stats_launch(self.h))
+
+    def flow_stats_ready(self):
+        r = C.c_int()
+        check(self.lib.felbm_flow_stats_ready(self.h, C.byref(r)))
+        return bool(r.value)
+
+    def flow_stats_collect(self):
+        out = np.empty(6)
+        check(self.lib.felbm_flow_stats_collect(self.h, self.slot, _dptr(out)))
+        return out
+
+    # :704-757 -> dict of rho, c, ux, uy, uz, p
+    def download(self):
+        out = np.empty(6 * self.n)
+        check(self.lib.felbm_download_fields(self.h, self.slot, _dptr(out)))
+        f = out.reshape(6, self.n)
+        return {"rho": f[0], "c": f[1], "ux": f[2], "uy": f[3], "uz": f[4], "p": f[5]}
+
+    def export_tables(self):
+        n = self.n
+        names = ("cdm", "cdp", "bcase", "bc1", "bc2", "avgnext", "lcase", "lc1", "lc2", "src")
+        sizes = (18, 18, 18, 18, 18, 19, 18, 18, 18, 19)
+        arrs = [np.empty(s * n, dtype=np.int32) for s in sizes]
+        check(self.lib.felbm_export_tables(self.h, self.slot,
+                                           *[a.ctypes.data_as(C.POINTER(C.c_int32)) for a in arrs]))
+        return dict(zip(names, arrs))
+
+    def free(self):
+        if self.slot is not None:
+            self.batch.remove(self.slot)
+            self.slot = None
+
+
+def mrt_matrices():
+    lib = _lib.load()
+    M = np.empty((19, 19))
+    Mi = np.empty((19, 19))
+    check(lib.felbm_mrt_matrices(_dptr(M), _dptr(Mi)))
+    return M, Mi

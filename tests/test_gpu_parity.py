@@ -1,0 +1,177 @@
+"""GPU parity: the HIP engine (through the C ABI) against the FP64 oracle.
+
+The bar: FP64 results bit-identical to the oracle (its folded mass-correction
+mode, mass_mode=2, which applies the reference's correction inside the next
+moments exactly as the engine does); FP32 within the stated tolerance of the
+FP64 oracle.  Mirrors the reference's compare_cpu_gpu harness
+(src/compare_cpu_gpu.cu:241-294): same initial distributions on both sides,
+distributions compared after N steps, then the fields.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from felbm_gpu_amd import MultiPhaseGPU, Batch
+from felbm_gpu_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small_pack(edge=24, por=0.55, seed=11):
+    return W.spheres(edge, edge, edge, 3.0, 5.0, por, seed)
+
+
+def make_pair(solid, params, use_mrt, mass, precision="f64", periodic=(1, 1, 1), hbb=True, c0=None):
+    from oracle.pyoracle import Oracle
+    o = Oracle(solid, params, use_mrt=use_mrt, fuse_collision=True, mass_mode=2 if mass else 0,
+               periodic=periodic, use_halfway_bb=hbb)
+    g = MultiPhaseGPU(precision=precision).init(solid, params, {"use_mrt": use_mrt, "correct_op_mass": mass},
+                                                 periodic=periodic, use_halfway_bb=hbb)
+    assert g.n == o.n
+    if c0 is None:
+        c0 = W.slab_c(solid, 0, 0, solid.shape[2] // 2, 4.0)
+    o.init_rest(c0, 1.0)
+    h, gg = o.download_state()
+    g.upload_state(h, gg)
+    o.record_target_mass()
+    g.record_target_mass()
+    return o, g
+
+
+def test_tables_bitwise_match_oracle():
+    from oracle.pyoracle import Oracle
+    cases = [("2d", W.cylinders(64, 48, 1, 6, 4.0, 7.0, seed=5)[0], True),
+             ("3d", small_pack(20, 0.5, 3), True),
+             ("fluid_nohbb", np.zeros((8, 9, 10), np.uint8), False)]
+    prm = W.derive_params()
+    for name, solid, hbb in cases:
+        _say("tables case", name, solid.shape)
+        o = Oracle(solid, prm, use_halfway_bb=hbb)
+        g = MultiPhaseGPU("f64").init(solid, prm, {}, use_halfway_bb=hbb)
+        to, tg = o.tables(), g.export_tables()
+        for k in to:
+            assert np.array_equal(to[k], tg[k]), (name, k)
+
+
+@pytest.mark.parametrize("use_mrt", [0, 1])
+def test_one_step_fields_and_state_bitexact(use_mrt):
+    w = W.workload("2d_cylinders_128")
+    _say("one step, mrt", use_mrt)
+    o, g = make_pair(w["solid"], w["params"], use_mrt, 1, c0=w["c0"])
+    o.run(1)
+    g.step(1)
+    ho, go = o.download_state()
+    hg, gg = g.download_state()
+    assert np.array_equal(ho, hg)
+    assert np.array_equal(go, gg)
+    fo, fg = o.download(), g.download()
+    for k in fo:
+        assert np.array_equal(fo[k], fg[k]), k
+
+
+@pytest.mark.parametrize("use_mrt", [0, 1])
+def test_20_steps_porous_3d_bitexact(use_mrt):
+    solid = small_pack()
+    prm = W.derive_params(theta_deg=45.0, acceleration=(1e-5, 0, 0), rho0=1.0, rho1=1.0)
+    _say("20 steps 3d, mrt", use_mrt, int((solid == 0).sum()))
+    o, g = make_pair(solid, prm, use_mrt, 1)
+    o.run(20)
+    g.step(20)
+    ho, go = o.download_state()
+    hg, gg = g.download_state()
+    assert np.isfinite(hg).all()
+    assert np.abs(ho - hg).max() == 0.0
+    assert np.abs(go - gg).max() == 0.0
+
+
+def test_density_ratio_bitexact():
+    """density ratio 5 (the reference's validate.sh cases) with MRT"""
+    solid = small_pack(20, 0.6, 4)
+    prm = W.derive_params(theta_deg=70.0, rho0=5.0, rho1=1.0, tau0=0.8, tau1=1.0, acceleration=(0, 1e-5, 0))
+    o, g = make_pair(solid, prm, 1, 1)
+    o.run(20)
+    g.step(20)
+    assert np.array_equal(o.download_state()[0], g.download_state()[0])
+    assert np.array_equal(o.download_state()[1], g.download_state()[1])
+
+
+def test_stats_and_mass_match():
+    solid = small_pack()
+    prm = W.derive_params(theta_deg=45.0, acceleration=(1e-5, 0, 0))
+    o, g = make_pair(solid, prm, 1, 1)
+    o.run(10)
+    g.step(10, stats_last=True)
+    so, sg = o.flow_stats(), g.flow_stats()
+    assert np.array_equal(so, sg), (so, sg)
+    assert o.current_mass() == g.current_mass()
+    assert abs(g.current_mass() - g.get_target_mass()) <= 1e-9 * abs(g.get_target_mass())
+
+
+def test_c1_full_2000_steps():
+    w = W.workload("2d_cylinders_128")
+    o, g = make_pair(w["solid"], w["params"], 0, 1, c0=w["c0"])
+    _say("C1 2000 steps, n =", g.n)
+    o.run(2000)
+    g.step(2000)
+    ho, go = o.download_state()
+    hg, gg = g.download_state()
+    dh, dg = np.abs(ho - hg).max(), np.abs(go - gg).max()
+    _say("C1 max|dh| max|dg|", dh, dg)
+    assert dh <= 1e-13 and dg <= 1e-13
+    fo, fg = o.download(), g.download()
+    for k in ("c", "rho", "ux", "uy", "uz", "p"):
+        assert np.abs(fo[k] - fg[k]).max() <= 1e-13, k
+
+
+def test_fp32_within_tolerance_of_fp64_oracle():
+    solid = small_pack(24, 0.55, 11)
+    prm = W.derive_params(theta_deg=45.0, acceleration=(1e-5, 0, 0))
+    o, g = make_pair(solid, prm, 1, 1, precision="f32")
+    o.run(100)
+    g.step(100)
+    fo, fg = o.download(), g.download()
+    dc = np.abs(fo["c"] - fg["c"]).max()
+    du = max(np.abs(fo[k] - fg[k]).max() for k in ("ux", "uy", "uz"))
+    _say("fp32 max|dc|", dc, "max|du|", du)
+    assert dc <= 1e-5 and du <= 1e-5
+
+
+def test_batch_equals_alone_and_graph_equals_eager():
+    prm = W.derive_params(theta_deg=60.0, acceleration=(1e-5, 0, 0))
+    solids = [small_pack(16, 0.6, 21), small_pack(20, 0.5, 22), W.cylinders(40, 30, 1, 4, 3.0, 6.0, seed=9)[0]]
+    sw = {"use_mrt": 1, "correct_op_mass": 1}
+    b = Batch(precision="f64")
+    mixed = [MultiPhaseGPU(batch=b).init(s, prm, sw) for s in solids]
+    alone = [MultiPhaseGPU("f64").init(s, prm, sw) for s in solids]
+    for s, m, a in zip(solids, mixed, alone):
+        c0 = W.slab_c(s, 0, 0, s.shape[2] // 2, 4.0)
+        m.init_rest(c0); a.init_rest(c0)
+        m.record_target_mass(); a.record_target_mass()
+    b.step_graph(4, 3)          # 12 steps through a captured graph
+    for a in alone:
+        a.step(12)
+    for m, a in zip(mixed, alone):
+        hm, gm = m.download_state()
+        ha, ga = a.download_state()
+        assert np.array_equal(hm, ha) and np.array_equal(gm, ga)
+
+
+def test_upload_download_roundtrip_and_restart():
+    solid = small_pack(16, 0.6, 31)
+    prm = W.derive_params()
+    o, g = make_pair(solid, prm, 1, 0)
+    h, gg = g.download_state()
+    ho, go = o.download_state()
+    assert np.array_equal(h, ho) and np.array_equal(gg, go)
+    g.step(5)
+    h1, g1 = g.download_state()
+    g2 = MultiPhaseGPU("f64").init(solid, prm, {"use_mrt": 1})
+    g2.upload_state(h1, g1)
+    g.step(5); g2.step(5)
+    assert np.array_equal(g.download_state()[0], g2.download_state()[0])
