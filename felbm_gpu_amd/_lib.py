@@ -81,6 +81,7 @@ SIGNATURES = {
     "felbm_export_tables": (_I, [_P, _I] + [_PI32] * 10),
     "felbm_mrt_matrices": (_I, [_PD, _PD]),
     "felbm_kernel_times": (_I, [_P, _I, C.POINTER(C.c_float)]),
+    "felbm_copy_probe": (_I, [_I, C.c_int64, _I, _PD]),
     "felbm_last_error": (C.c_char_p, []),
     "felbm_version": (C.c_char_p, []),
     "felbm_geom_cylinders": (_I, [_I, _I, _I, _I, _D, _D, _D, C.c_uint64, _I, C.POINTER(C.c_uint8)]),

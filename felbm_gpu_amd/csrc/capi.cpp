@@ -86,3 +86,32 @@ int felbm_mrt_matrices(double *M, double *Minv)
 }
 
 }  // extern "C"
+
+extern "C" int felbm_copy_probe(int device, int64_t bytes, int reps, double *gbs)
+{
+  // device-to-device copy of `bytes`, `reps` times: the achievable streaming
+  // bandwidth (reads + writes) and the calibration kernel for HBM counters
+  if (!gbs || bytes <= 0 || reps <= 0) { fm::set_error("invalid copy probe arguments"); return FELBM_EINVAL; }
+  if (hipSetDevice(device) != hipSuccess) { fm::set_error("hipSetDevice failed"); return FELBM_EHIP; }
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, (size_t)bytes) != hipSuccess) { fm::set_error("allocation failed"); return FELBM_ENOMEM; }
+  if (hipMalloc(&b, (size_t)bytes) != hipSuccess) { (void)hipFree(a); fm::set_error("allocation failed"); return FELBM_ENOMEM; }
+  hipStream_t s; hipEvent_t e0, e1;
+  int rc = FELBM_OK;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) rc = FELBM_EHIP;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = FELBM_EHIP;
+  if (!rc) {
+    (void)hipMemsetAsync(a, 1, (size_t)bytes, s);
+    (void)hipMemcpyAsync(b, a, (size_t)bytes, hipMemcpyDeviceToDevice, s);
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; ++r) (void)hipMemcpyAsync(b, a, (size_t)bytes, hipMemcpyDeviceToDevice, s);
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = FELBM_EHIP;
+    float ms = 0; (void)hipEventElapsedTime(&ms, e0, e1);
+    *gbs = ms > 0 ? 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9 : 0.0;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipStreamDestroy(s);
+  }
+  (void)hipFree(a); (void)hipFree(b);
+  if (rc) fm::set_error("copy probe failed");
+  return rc;
+}

@@ -2,7 +2,7 @@
 #pragma once
 #include <cstdint>
 #include <string>
-#include "../../include/felbm_mi355x.h"
+#include "felbm_mi355x.h"
 
 namespace fm {
 

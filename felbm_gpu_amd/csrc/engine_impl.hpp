@@ -14,6 +14,7 @@
 #include <algorithm>
 #include "engine.hpp"
 #include "kernels.hpp"
+#include "collide_f32.hpp"
 
 namespace fm {
 
@@ -31,7 +32,7 @@ struct Slot {
   int n = 0, ns = 0, nblocks = 0;
   int Nx = 0, Ny = 0, Nz = 0, wpr = 0;
   int per[3] = {1, 1, 1};
-  int hbb = 1, mrt = 0, mass_on = 0;
+  int hbb = 1, mrt = 0, mass_on = 0, open = 0;
   int par0 = 0;
   double M0 = 0.0;
   double params[32];
@@ -102,6 +103,10 @@ public:
     if (cnt == 0) { set_error("geometry has no fluid sites"); return FELBM_EINVAL; }
     if (cnt >= (int64_t)1 << 31) { set_error("more than 2^31 sites"); return FELBM_EUNSUPPORTED; }
     S.n = (int)cnt; S.ns = (int)align_up((size_t)cnt, 64); S.wpr = wpr;
+    if ((uint64_t)19 * (uint64_t)S.ns * sizeof(real) >= ((uint64_t)1 << 32) || words.size() * 8 >= ((size_t)1 << 32)) {
+      set_error("instance too large: 19*sites*sizeof(real) must stay below 4 GiB");
+      return FELBM_EUNSUPPORTED;
+    }
     S.nblocks = (S.n + 255) / 256;
     // one allocation: h, g0, g1 (19 ns), 9 fields + 4 export (ns), words, coords, acc
     const size_t RB = sizeof(real), ns = (size_t)S.ns;
@@ -123,6 +128,8 @@ public:
     d.px = S.per[0]; d.py = S.per[1]; d.pz = S.per[2];
     d.hbb = S.hbb; d.mrt = S.mrt; d.mass_on = S.mass_on;
     d.words = (const uint64_t *)(S.base + o_w); d.coords = (const uint32_t *)(S.base + o_c);
+    d.nwords = (unsigned)words.size();
+    S.open = !(S.per[0] && S.per[1] && S.per[2]);
     d.h = (real *)(S.base + o_h); d.g0 = (real *)(S.base + o_g0); d.g1 = (real *)(S.base + o_g1);
     real *f = (real *)(S.base + o_f);
     d.craw = f; d.p0 = f + ns; d.jx = f + 2 * ns; d.jy = f + 3 * ns; d.jz = f + 4 * ns;
@@ -390,9 +397,12 @@ private:
   std::vector<Slot<real>> slots_;
   std::vector<int> slot_desc_;      // slot -> descriptor index
   Inst<real> *d_desc_ = nullptr; int desc_cap_ = 0;
-  int2 *d_map_ = nullptr, *d_map_mrt_ = nullptr, *d_map_bgk_ = nullptr;
+  int2 *d_map_ = nullptr;
+  int2 *d_var_[8] = {};
+  int nvar_[8] = {};
   int map_cap_ = 0;
-  int nmap_ = 0, nmap_mrt_ = 0, nmap_bgk_ = 0;
+  int nmap_ = 0;
+  static int variant(const Slot<real> &s) { return (s.mrt ? 1 : 0) | (s.hbb ? 2 : 0) | (s.open ? 4 : 0); }
   bool dirty_ = false, dirty_desc_ = false;
   unsigned T_ = 0;
   unsigned map_ver_ = 0;
@@ -437,22 +447,24 @@ private:
   int rebuild() {
     slot_desc_.assign(slots_.size(), -1);
     std::vector<Inst<real>> descs;
-    std::vector<int2> all, mrt, bgk;
+    std::vector<int2> all, var[8];
     for (size_t q = 0; q < slots_.size(); ++q) {
       if (!slots_[q].used) continue;
       int di = (int)descs.size();
       slot_desc_[q] = di;
       descs.push_back(slots_[q].d);
+      const int key = variant(slots_[q]);
       for (int b = 0; b < slots_[q].nblocks; ++b) {
         int2 e = make_int2(di, b * 256);
         all.push_back(e);
-        (slots_[q].mrt ? mrt : bgk).push_back(e);
+        var[key].push_back(e);
       }
     }
     auto pad = [](std::vector<int2> &v) { while (v.size() % 8) v.push_back(make_int2(-1, 0)); };
-    pad(all); pad(mrt); pad(bgk);
-    nmap_ = (int)all.size(); nmap_mrt_ = (int)mrt.size(); nmap_bgk_ = (int)bgk.size();
-    int need = nmap_ + nmap_mrt_ + nmap_bgk_;
+    pad(all); for (auto &v : var) pad(v);
+    nmap_ = (int)all.size();
+    int need = nmap_;
+    for (int k = 0; k < 8; ++k) { nvar_[k] = (int)var[k].size(); need += nvar_[k]; }
     if ((int)descs.size() > desc_cap_) {
       if (d_desc_) FM_HIP(hipFree(d_desc_));
       desc_cap_ = std::max((int)descs.size(), 2 * desc_cap_);
@@ -463,11 +475,16 @@ private:
       map_cap_ = std::max(need, 2 * map_cap_);
       FM_HIP(hipMalloc((void **)&d_map_, sizeof(int2) * map_cap_));
     }
-    d_map_mrt_ = d_map_ + nmap_; d_map_bgk_ = d_map_mrt_ + nmap_mrt_;
     if (!descs.empty()) FM_HIP(hipMemcpyAsync(d_desc_, descs.data(), sizeof(Inst<real>) * descs.size(), hipMemcpyHostToDevice, stream_));
     if (nmap_) FM_HIP(hipMemcpyAsync(d_map_, all.data(), sizeof(int2) * nmap_, hipMemcpyHostToDevice, stream_));
-    if (nmap_mrt_) FM_HIP(hipMemcpyAsync(d_map_mrt_, mrt.data(), sizeof(int2) * nmap_mrt_, hipMemcpyHostToDevice, stream_));
-    if (nmap_bgk_) FM_HIP(hipMemcpyAsync(d_map_bgk_, bgk.data(), sizeof(int2) * nmap_bgk_, hipMemcpyHostToDevice, stream_));
+    {
+      int2 *p = d_map_ + nmap_;
+      for (int k = 0; k < 8; ++k) {
+        d_var_[k] = p;
+        if (nvar_[k]) FM_HIP(hipMemcpyAsync(p, var[k].data(), sizeof(int2) * nvar_[k], hipMemcpyHostToDevice, stream_));
+        p += nvar_[k];
+      }
+    }
     // single-instance maps (field downloads) carry the new descriptor indices
     std::vector<std::vector<int2>> single(slots_.size());
     for (size_t q = 0; q < slots_.size(); ++q) {
@@ -502,18 +519,43 @@ private:
     return FELBM_OK;
   }
 
+  template <bool MRT, bool HBB, bool OPEN>
+  void launch_collide(dim3 grid, const int2 *map, bool stats) {
+    if constexpr (sizeof(real) == 4)
+      hipLaunchKernelGGL((k_collide_f32<MRT, HBB, OPEN>), grid, dim3(256), 0, stream_, (const Inst<float> *)d_desc_, map, T_, stats ? 1 : 0);
+    else
+      hipLaunchKernelGGL((k_collide<real, MRT, HBB, OPEN>), grid, dim3(256), 0, stream_, d_desc_, map, T_, stats ? 1 : 0);
+  }
+  template <bool MRT, bool HBB, bool OPEN>
+  void launch_variant(int stage, dim3 grid, const int2 *map, bool stats, real *exp, int exp_inst) {
+    if (stage == 2) hipLaunchKernelGGL((k_cfield<real, HBB, OPEN>), grid, dim3(256), 0, stream_, d_desc_, map, T_, exp, exp_inst);
+    else if (stage == 3) hipLaunchKernelGGL((k_mufield<real, HBB, OPEN>), grid, dim3(256), 0, stream_, d_desc_, map, T_);
+    else launch_collide<MRT, HBB, OPEN>(grid, map, stats);
+  }
+  void launch_key(int key, int stage, dim3 grid, const int2 *map, bool stats, real *exp = nullptr, int exp_inst = -1) {
+    switch (key) {
+      case 0: launch_variant<false, false, false>(stage, grid, map, stats, exp, exp_inst); break;
+      case 1: launch_variant<true,  false, false>(stage, grid, map, stats, exp, exp_inst); break;
+      case 2: launch_variant<false, true,  false>(stage, grid, map, stats, exp, exp_inst); break;
+      case 3: launch_variant<true,  true,  false>(stage, grid, map, stats, exp, exp_inst); break;
+      case 4: launch_variant<false, false, true >(stage, grid, map, stats, exp, exp_inst); break;
+      case 5: launch_variant<true,  false, true >(stage, grid, map, stats, exp, exp_inst); break;
+      case 6: launch_variant<false, true,  true >(stage, grid, map, stats, exp, exp_inst); break;
+      default: launch_variant<true,  true,  true >(stage, grid, map, stats, exp, exp_inst); break;
+    }
+  }
+
   int launch_step(bool stats, bool timed = false) {
     if (stats) if (int e = zero_stats()) return e;
     dim3 blk(256);
     if (timed) FM_HIP(hipEventRecord(ev_[0], stream_));
     hipLaunchKernelGGL(k_moments<real>, dim3(nmap_), blk, 0, stream_, d_desc_, d_map_, T_); FM_LAUNCH();
     if (timed) FM_HIP(hipEventRecord(ev_[1], stream_));
-    hipLaunchKernelGGL(k_cfield<real>, dim3(nmap_), blk, 0, stream_, d_desc_, d_map_, T_, (real *)nullptr, -1); FM_LAUNCH();
+    for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_key(k, 2, dim3(nvar_[k]), d_var_[k], stats); FM_LAUNCH(); }
     if (timed) FM_HIP(hipEventRecord(ev_[2], stream_));
-    hipLaunchKernelGGL(k_mufield<real>, dim3(nmap_), blk, 0, stream_, d_desc_, d_map_, T_); FM_LAUNCH();
+    for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_key(k, 3, dim3(nvar_[k]), d_var_[k], stats); FM_LAUNCH(); }
     if (timed) FM_HIP(hipEventRecord(ev_[3], stream_));
-    if (nmap_mrt_) { hipLaunchKernelGGL((k_collide<real, true>), dim3(nmap_mrt_), blk, 0, stream_, d_desc_, d_map_mrt_, T_, stats ? 1 : 0); FM_LAUNCH(); }
-    if (nmap_bgk_) { hipLaunchKernelGGL((k_collide<real, false>), dim3(nmap_bgk_), blk, 0, stream_, d_desc_, d_map_bgk_, T_, stats ? 1 : 0); FM_LAUNCH(); }
+    for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_key(k, 4, dim3(nvar_[k]), d_var_[k], stats); FM_LAUNCH(); }
     if (timed) FM_HIP(hipEventRecord(ev_[4], stream_));
     ++T_;
     return FELBM_OK;
@@ -565,7 +607,7 @@ private:
     const size_t ns = (size_t)S.ns;
     dim3 blk(256), grid(S.dmap_len);
     hipLaunchKernelGGL(k_moments<real>, grid, blk, 0, stream_, d_desc_, S.dmap, T_); FM_LAUNCH();
-    hipLaunchKernelGGL(k_cfield<real>, grid, blk, 0, stream_, d_desc_, S.dmap, T_, S.d.craw + export_off_ * ns, di); FM_LAUNCH();
+    launch_key(variant(S), 2, grid, S.dmap, false, S.d.craw + export_off_ * ns, di); FM_LAUNCH();
     // the step's own moments pass must find this parity's partials at zero
     unsigned cur = cur_of(S);
     FM_HIP(hipMemsetAsync(S.d.acc + ACC_MASS + cur * 5 * NSLOT, 0, 5 * NSLOT * 8, stream_));

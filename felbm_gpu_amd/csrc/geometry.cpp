@@ -14,7 +14,7 @@
 #include <cmath>
 #include <vector>
 #include <numeric>
-#include "../../include/felbm_mi355x.h"
+#include "felbm_mi355x.h"
 
 namespace {
 

@@ -48,7 +48,7 @@ constexpr double WD[19] = { 1.0/3.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0
 // pxz, mx, my, mz -- the order the relaxation vector of device_engine.cuh:625
 // assumes.  Minv = M^T diag(1/|row|^2).  Compile-time, so zero entries fold
 // away from the unrolled transform without changing any rounding.
-struct MrtTab { double M[19][19]; double MI[19][19]; };
+struct MrtTab { double M[19][19]; double MI[19][19]; double N[19]; };
 constexpr MrtTab make_mrt()
 {
   MrtTab t{};
@@ -64,6 +64,7 @@ constexpr MrtTab make_mrt()
   for (int m = 0; m < 19; ++m) {
     double nrm = 0;
     for (int k = 0; k < 19; ++k) nrm += t.M[m][k]*t.M[m][k];
+    t.N[m] = nrm;
     for (int k = 0; k < 19; ++k) t.MI[k][m] = t.M[m][k] / nrm;
   }
   return t;
@@ -81,6 +82,7 @@ struct Inst {
   int hbb, mrt, mass_on, par0;
   const uint64_t *words;
   const uint32_t *coords;
+  unsigned nwords;
   real *h, *g0, *g1;
   real *craw, *p0, *jx, *jy, *jz;
   real *c, *mu, *ps, *lapmu;
@@ -163,45 +165,146 @@ __device__ __forceinline__ int logical_block()
   return (int)((b & 7u) * (nb >> 3) + (b >> 3));      // nb is a multiple of 8
 }
 
-// ---- lattice lookup ---------------------------------------------------------------
-// site at (x,y,z) (|offset| <= 2 from a valid voxel): >= 0 site index,
-// -1 solid voxel, -2 outside a non-periodic face.
-template <typename real>
-__device__ __forceinline__ int site_at(const Inst<real> &d, int x, int y, int z)
+// ---- memory access ------------------------------------------------------------------
+// Generic pointers from the descriptor are cast to address space 1 for plain
+// global accesses; the SoA arrays go through raw buffer loads/stores so the
+// per-direction offset k*ns lives in a scalar register (no VALU address math).
+template <typename T>
+__device__ __forceinline__ T ldg(const T *p, long i) { return ((const __attribute__((address_space(1))) T *)p)[i]; }
+template <typename T>
+__device__ __forceinline__ void stg(T *p, long i, T v) { ((__attribute__((address_space(1))) T *)p)[i] = v; }
+
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+// raw buffer resource: base, stride 0, num_records in bytes, gfx9 data format word
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, unsigned bytes)
 {
-  if (x < 0)        { if (!d.px) return -2; x += d.Nx; if (x < 0) x += d.Nx; }
-  else if (x >= d.Nx) { if (!d.px) return -2; x -= d.Nx; if (x >= d.Nx) x -= d.Nx; }
-  if (y < 0)        { if (!d.py) return -2; y += d.Ny; if (y < 0) y += d.Ny; }
-  else if (y >= d.Ny) { if (!d.py) return -2; y -= d.Ny; if (y >= d.Ny) y -= d.Ny; }
-  if (z < 0)        { if (!d.pz) return -2; z += d.Nz; if (z < 0) z += d.Nz; }
-  else if (z >= d.Nz) { if (!d.pz) return -2; z -= d.Nz; if (z >= d.Nz) z -= d.Nz; }
-  uint64_t w = d.words[((unsigned)z*(unsigned)d.Ny + (unsigned)y)*(unsigned)d.wpr + ((unsigned)x >> 5)];
-  unsigned b = (unsigned)x & 31u, m = (unsigned)w;
-  if (!((m >> b) & 1u)) return -1;
-  return (int)(unsigned)(w >> 32) + __popc(m & ((1u << b) - 1u));
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T> __device__ __forceinline__ T bld(rsrc_t r, unsigned vo, unsigned so);
+template <> __device__ __forceinline__ float bld<float>(rsrc_t r, unsigned vo, unsigned so)
+{ return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0)); }
+template <> __device__ __forceinline__ double bld<double>(rsrc_t r, unsigned vo, unsigned so)
+{ return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0)); }
+template <> __device__ __forceinline__ uint64_t bld<uint64_t>(rsrc_t r, unsigned vo, unsigned so)
+{ return __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0)); }
+__device__ __forceinline__ void bst(float v, rsrc_t r, unsigned vo, unsigned so)
+{ __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vo, (int)so, 0); }
+__device__ __forceinline__ void bst(double v, rsrc_t r, unsigned vo, unsigned so)
+{ __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, (int)vo, (int)so, 0); }
+
+// field block layout: [craw, p0, jx, jy, jz, c, mu, ps, lapmu] x ns
+enum { F_CRAW = 0, F_P0 = 1, F_JX = 2, F_JY = 3, F_JZ = 4, F_C = 5, F_MU = 6, F_PS = 7, F_LAPMU = 8, F_COUNT = 13 };
+
+// ---- lattice lookup ---------------------------------------------------------------
+// The wrapped coordinates x-2..x+2 (and y, z) of one site, computed once;
+// bit a*5+j of ok says whether offset j-2 on axis a exists (always on a
+// periodic axis).  Out-of-box coordinates are still wrapped, so every word
+// load stays inside the table.
+struct Nbhd { int xw[5], yw[5], zw[5]; unsigned ok; };
+
+__device__ __forceinline__ int wrap_axis(int v, int N, int per, bool &ok)
+{
+  ok = per || (v >= 0 && v < N);
+  v = v < 0 ? v + N : v; v = v < 0 ? v + N : v;
+  v = v >= N ? v - N : v; v = v >= N ? v - N : v;
+  return v;
 }
 
 template <typename real>
 __device__ __forceinline__ void decode(const Inst<real> &d, int i, int &x, int &y, int &z)
 {
-  uint32_t c = d.coords[i];
+  uint32_t c = ldg(d.coords, i);
   x = (int)(c & d.mx); y = (int)((c >> d.sx) & d.my); z = (int)(c >> d.sxy);
 }
 
-// neighbour sites at x + e_m (r = 1) for m = 1..18; n1[0] = i
 template <typename real>
-__device__ __forceinline__ void neighbours1(const Inst<real> &d, int i, int x, int y, int z, int (&n1)[Q])
+__device__ __forceinline__ void make_nbhd(const Inst<real> &d, int x, int y, int z, Nbhd &nb)
+{
+  nb.ok = 0;
+  #pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    bool a, b, c;
+    nb.xw[j] = wrap_axis(x + j - 2, d.Nx, d.px, a);
+    nb.yw[j] = wrap_axis(y + j - 2, d.Ny, d.py, b);
+    nb.zw[j] = wrap_axis(z + j - 2, d.Nz, d.pz, c);
+    nb.ok |= (a ? 1u << j : 0u) | (b ? 1u << (5 + j) : 0u) | (c ? 1u << (10 + j) : 0u);
+  }
+}
+
+// The voxel at offset (dx,dy,dz), |d| <= 2.  Returns the byte offset (times
+// RB) of its site, or of the centre site itself when it is not a site, and sets
+// fluid / solid-inside-the-box flags.  Branch-free.
+template <typename real, bool OPEN>
+__device__ __forceinline__ unsigned nb_lookup(const Inst<real> &d, rsrc_t rW, const Nbhd &nb,
+                                              int dx, int dy, int dz, unsigned self_off,
+                                              bool &fluid, bool &solid)
+{
+  const int xx = nb.xw[dx + 2], yy = nb.yw[dy + 2], zz = nb.zw[dz + 2];
+  const unsigned widx = ((unsigned)zz*(unsigned)d.Ny + (unsigned)yy)*(unsigned)d.wpr + ((unsigned)xx >> 5);
+  const uint64_t w = bld<uint64_t>(rW, widx*8u, 0u);
+  const unsigned lo = (unsigned)w, b = (unsigned)xx & 31u;
+  const unsigned idx = (unsigned)(w >> 32) + (unsigned)__popc(lo & ((1u << b) - 1u));
+  bool f = (lo >> b) & 1u;
+  bool inbox = true;
+  if constexpr (OPEN) {
+    const unsigned bits = (1u << (dx + 2)) | (1u << (7 + dy)) | (1u << (12 + dz));
+    inbox = (nb.ok & bits) == bits;
+  }
+  fluid = f && inbox;
+  solid = !f && inbox;
+  return fluid ? idx*(unsigned)sizeof(real) : self_off;
+}
+
+// r = 1 ring: byte offsets vo[m] of x + e_m (own offset when not a site),
+// fluid bits fl and solid-in-box bits so (bit m)
+template <typename real, bool OPEN>
+__device__ __forceinline__ void ring1(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
+                                      unsigned (&vo)[Q], unsigned &fl, unsigned &so)
+{
+  vo[0] = self_off; fl = 1u; so = 0u;
+  #pragma unroll
+  for (int m = 1; m < Q; ++m) {
+    bool f, s;
+    vo[m] = nb_lookup<real, OPEN>(d, rW, nb, CX[m], CY[m], CZ[m], self_off, f, s);
+    fl |= f ? (1u << m) : 0u; so |= s ? (1u << m) : 0u;
+  }
+}
+// r = 2 ring (x + 2 e_m): offsets and fluid bits
+template <typename real, bool OPEN>
+__device__ __forceinline__ void ring2(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
+                                      unsigned (&vo)[Q], unsigned &fl)
+{
+  vo[0] = self_off; fl = 0u;
+  #pragma unroll
+  for (int m = 1; m < Q; ++m) {
+    bool f, s;
+    vo[m] = nb_lookup<real, OPEN>(d, rW, nb, 2*CX[m], 2*CY[m], 2*CZ[m], self_off, f, s);
+    fl |= f ? (1u << m) : 0u;
+  }
+}
+
+// site codes for table export: >= 0 site, -1 solid voxel, -2 outside the box
+template <typename real>
+__device__ __forceinline__ int nb_site(const Inst<real> &d, const Nbhd &nb, int dx, int dy, int dz)
+{
+  rsrc_t rW = make_rsrc(d.words, d.nwords*8u);
+  bool f, s;
+  unsigned off = nb_lookup<real, true>(d, rW, nb, dx, dy, dz, 0xffffffffu, f, s);
+  return f ? (int)(off / (unsigned)sizeof(real)) : (s ? -1 : -2);
+}
+template <typename real>
+__device__ __forceinline__ void neighbours1(const Inst<real> &d, const Nbhd &nb, int i, int (&n1)[Q])
 {
   n1[0] = i;
   #pragma unroll
-  for (int m = 1; m < Q; ++m) n1[m] = site_at(d, x + CX[m], y + CY[m], z + CZ[m]);
+  for (int m = 1; m < Q; ++m) n1[m] = nb_site(d, nb, CX[m], CY[m], CZ[m]);
 }
 template <typename real>
-__device__ __forceinline__ void neighbours2(const Inst<real> &d, int x, int y, int z, int (&n2)[Q])
+__device__ __forceinline__ void neighbours2(const Inst<real> &d, const Nbhd &nb, int (&n2)[Q])
 {
   n2[0] = -1;
   #pragma unroll
-  for (int m = 1; m < Q; ++m) n2[m] = site_at(d, x + 2*CX[m], y + 2*CY[m], z + 2*CZ[m]);
+  for (int m = 1; m < Q; ++m) n2[m] = nb_site(d, nb, 2*CX[m], 2*CY[m], 2*CZ[m]);
 }
 
 template <typename real> __device__ __forceinline__ real W(int k) { return (real)WD[k]; }
@@ -213,8 +316,8 @@ __device__ __forceinline__ double mass_lambda(const Inst<real> &d, unsigned cur,
 {
   const unsigned long long *a = d.acc + ACC_MASS + cur*5*NSLOT;
   int lane = threadIdx.x & 63;
-  unsigned long long mh = wave_sum(a[0*NSLOT + lane]), ml = wave_sum(a[1*NSLOT + lane]);
-  unsigned long long wh = wave_sum(a[2*NSLOT + lane]), wl = wave_sum(a[3*NSLOT + lane]);
+  unsigned long long mh = wave_sum(ldg(a, 0*NSLOT + lane)), ml = wave_sum(ldg(a, 1*NSLOT + lane));
+  unsigned long long wh = wave_sum(ldg(a, 2*NSLOT + lane)), wl = wave_sum(ldg(a, 3*NSLOT + lane));
   double M = fx_value((long long)mh, ml), Wt = fx_value((long long)wh, wl);
   double drift = M - d.M0;
   apply = (Wt > 0.0 && drift != 0.0);
@@ -226,12 +329,17 @@ __device__ __forceinline__ double mass_lambda(const Inst<real> &d, unsigned cur,
 template <typename real>
 __device__ __forceinline__ real corrected(real c, bool apply, double lambda)
 {
-  if (apply) {
-    double cm = (double)c;
-    double phi = cm*(1.0 - cm);
-    if (phi > 0.0) return (real)(cm + (-lambda*phi));
-  }
-  return c;
+  double cm = (double)c;
+  double phi = cm*(1.0 - cm);
+  return (apply && phi > 0.0) ? (real)(cm + (-lambda*phi)) : c;
+}
+
+template <typename real>
+__device__ __forceinline__ void acc_vec(int m, real t, real &sx, real &sy, real &sz)
+{
+  if (CX[m] > 0) sx += t; else if (CX[m] < 0) sx -= t;
+  if (CY[m] > 0) sy += t; else if (CY[m] < 0) sy -= t;
+  if (CZ[m] > 0) sz += t; else if (CZ[m] < 0) sz -= t;
 }
 
 // ==== K1: moments (k_moments, device_engine.cuh:116-135, pass 1 only) ===========
@@ -244,24 +352,26 @@ __global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ 
   const Inst<real> &d = D[bm.x];
   const int i = bm.y + (int)threadIdx.x;
   const unsigned cur = (T ^ (unsigned)d.par0) & 1u;
-  const real *g = cur ? d.g1 : d.g0;
-  const int ns = d.ns;
+  constexpr unsigned RB = sizeof(real);
+  const unsigned nsb = (unsigned)d.ns*RB;
+  const rsrc_t rH = make_rsrc(d.h, 19u*nsb), rG = make_rsrc(cur ? d.g1 : d.g0, 19u*nsb);
+  const rsrc_t rF = make_rsrc(d.craw, (unsigned)F_COUNT*nsb);
   if (bm.y == 0) {                                  // clear the other parity's partials
     unsigned long long *o = d.acc + ACC_MASS + (cur ^ 1u)*5*NSLOT;
-    for (int q = threadIdx.x; q < 5*NSLOT; q += blockDim.x) o[q] = 0ull;
+    for (int q = threadIdx.x; q < 5*NSLOT; q += blockDim.x) stg(o, q, 0ull);
   }
   Fx m = {0, 0, 0}, w = {0, 0, 0};
   if (i < d.n) {
+    const unsigned io = (unsigned)i*RB;
     real c = 0, p = 0, ux = 0, uy = 0, uz = 0;
     #pragma unroll
     for (int k = 0; k < Q; ++k) {
-      real hv = d.h[k*ns + i], gv = g[k*ns + i];
+      real hv = bld<real>(rH, io, (unsigned)k*nsb), gv = bld<real>(rG, io, (unsigned)k*nsb);
       c += hv; p += gv;
-      if (CX[k] > 0) ux += gv; else if (CX[k] < 0) ux -= gv;
-      if (CY[k] > 0) uy += gv; else if (CY[k] < 0) uy -= gv;
-      if (CZ[k] > 0) uz += gv; else if (CZ[k] < 0) uz -= gv;
+      acc_vec<real>(k, gv, ux, uy, uz);
     }
-    d.craw[i] = c; d.p0[i] = p; d.jx[i] = ux; d.jy[i] = uy; d.jz[i] = uz;
+    bst(c, rF, io, F_CRAW*nsb); bst(p, rF, io, F_P0*nsb);
+    bst(ux, rF, io, F_JX*nsb); bst(uy, rF, io, F_JY*nsb); bst(uz, rF, io, F_JZ*nsb);
     if (d.mass_on) {              // k_mass_weight, device_engine.cuh:1109-1126
       double cm = (double)c;
       double phi = cm*(1.0 - cm); if (phi < 0.0) phi = 0.0;
@@ -274,106 +384,52 @@ __global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ 
   }
 }
 
-// ---- shared stencil pieces -------------------------------------------------------
-// central-difference vector gradient (k_grad_cd_mf, device_engine.cuh:897-913)
-// with the near-wall cases of multiphase_gpu.cuh:113-121.
-template <typename real, typename F>
-__device__ __forceinline__ void grad_cd(const int (&n1)[Q], int hbb, real f0, F f,
-                                        real &sx, real &sy, real &sz)
+// ---- per-direction stencil values ---------------------------------------------------
+// Neighbour values are fetched at the own site when the neighbour is not a site,
+// which makes the halfway-bounce-back central difference f(x+e)-f(x-e) already
+// equal to the reference's backward / forward / none cases (multiphase_gpu.cuh:
+// 113-121): f0-f(x-e), f(x+e)-f0 and f0-f0 = 0.
+template <typename real, bool HBB>
+__device__ __forceinline__ real cd_num(bool hin, bool hip, real fin, real fip, bool &use)
 {
-  sx = 0; sy = 0; sz = 0;
-  #pragma unroll
-  for (int m = 1; m < Q; ++m) {
-    int in = n1[m], ip = n1[OPP[m]];
-    real dd;
-    if (in >= 0 && ip >= 0) dd = f(in) - f(ip);
-    else if (hbb && ip >= 0) dd = f0 - f(ip);
-    else if (hbb && in >= 0) dd = f(in) - f0;
-    else continue;
-    real coef = (real)3.0*(real)0.5*W<real>(m);
-    real t = coef*dd;
-    if (CX[m] > 0) sx += t; else if (CX[m] < 0) sx -= t;
-    if (CY[m] > 0) sy += t; else if (CY[m] < 0) sy -= t;
-    if (CZ[m] > 0) sz += t; else if (CZ[m] < 0) sz -= t;
+  if constexpr (HBB) { use = true; return fin - fip; }
+  use = hin && hip;
+  return fin - fip;
+}
+// biased value (bd_dir_val, device_engine.cuh:96-107; cases multiphase_gpu.cuh:140-153)
+template <typename real, bool HBB>
+__device__ __forceinline__ real bd_num(bool hin, bool hin2, bool hip, bool hip2,
+                                       real fin, real fin2, real fip, real fip2, real f0)
+{
+  if constexpr (HBB) {
+    real v3 = (real)(-1.5)*f0 + (real)2.0*fin + (real)(-0.5)*fin2;
+    real vf = (real)(-1.5)*f0 + (real)1.5*fin;
+    real vb = (real)0.5*f0 + (real)(-0.5)*fip;
+    return hin ? (hin2 ? v3 : vf) : (hip ? vb : (real)0);
+  } else {
+    real f1 = hin ? fin : fip, f2 = hin ? fin2 : fip2;
+    real v3 = (real)(-1.5)*f0 + (real)2.0*f1 + (real)(-0.5)*f2;
+    real vf = (real)(-2.0)*f0 + (real)2.0*fin;
+    return hin ? (hin2 ? v3 : vf) : ((hip && hip2) ? v3 : (real)0);
   }
 }
-
-// biased per-direction value (bd_dir_val, device_engine.cuh:96-107) with the
-// cases of multiphase_gpu.cuh:140-153
-template <typename real, typename F>
-__device__ __forceinline__ real bd_val(const int (&n1)[Q], const int (&n2)[Q], int hbb, int m, real f0, F f)
+// Laplacian contribution (k_lap_mf, device_engine.cuh:996-1010; cases :178-188)
+template <typename real, bool HBB>
+__device__ __forceinline__ real lap_term(bool hin, bool hip, real fin, real fip, real f0, real bnd, real w)
 {
-  int in = n1[m], ip = n1[OPP[m]];
-  if (in >= 0) {
-    if (n2[m] >= 0) { real f1 = f(in), f2 = f(n2[m]); return (real)(-1.5)*f0 + (real)2.0*f1 + (real)(-0.5)*f2; }
-    real f1 = f(in);
-    if (!hbb) return (real)(-2.0)*f0 + (real)2.0*f1;
-    return (real)(-1.5)*f0 + (real)1.5*f1;
-  }
-  if (!hbb) {
-    if (ip >= 0 && n2[OPP[m]] >= 0) { real f1 = f(ip), f2 = f(n2[OPP[m]]); return (real)(-1.5)*f0 + (real)2.0*f1 + (real)(-0.5)*f2; }
-    return (real)0;
-  }
-  if (ip >= 0) { real f1 = f(ip); return (real)0.5*f0 + (real)(-0.5)*f1; }
-  return (real)0;
-}
-template <typename real>
-__device__ __forceinline__ bool bd_has(const int (&n1)[Q], int hbb, int m)
-{
-  int in = n1[m], ip = n1[OPP[m]];
-  if (in >= 0) return true;
-  if (!hbb) return false;     // non-hbb backward needs the 2-step check; handled by value 0
-  return ip >= 0;
-}
-
-// biased vector gradient (k_grad_bd_mf, device_engine.cuh:924-945)
-template <typename real, typename F>
-__device__ __forceinline__ void grad_bd(const int (&n1)[Q], const int (&n2)[Q], int hbb, real f0, F f,
-                                        real &sx, real &sy, real &sz)
-{
-  sx = 0; sy = 0; sz = 0;
-  #pragma unroll
-  for (int m = 1; m < Q; ++m) {
-    int in = n1[m], ip = n1[OPP[m]];
-    bool has;
-    if (in >= 0) has = true;
-    else if (!hbb) has = (ip >= 0 && n2[OPP[m]] >= 0);
-    else has = (ip >= 0);
-    if (!has) continue;
-    real val = bd_val<real>(n1, n2, hbb, m, f0, f);
-    real base = (real)3.0*W<real>(m);
-    real t = base*val;
-    if (CX[m] > 0) sx += t; else if (CX[m] < 0) sx -= t;
-    if (CY[m] > 0) sy += t; else if (CY[m] < 0) sy -= t;
-    if (CZ[m] > 0) sz += t; else if (CZ[m] < 0) sz -= t;
-  }
-}
-
-// node-centred Laplacian (k_lap_mf, device_engine.cuh:996-1010) with the cases
-// of multiphase_gpu.cuh:178-188; bnd = the second half of the 2n input
-template <typename real, typename F>
-__device__ __forceinline__ real laplacian(const int (&n1)[Q], int hbb, real f0, real bnd, F f)
-{
-  real s = 0;
-  #pragma unroll
-  for (int m = 1; m < Q; ++m) {
-    int in = n1[m], ip = n1[OPP[m]];
-    real w = (real)3.0*W<real>(m);
-    if (in >= 0 && ip >= 0) { real f1 = f(ip), f2 = f(in); s += w*(f1 - (real)2*f0 + f2); }
-    else if (ip >= 0 || in >= 0) {
-      real f1 = f(ip >= 0 ? ip : in);
-      if (hbb) s += w*(f1 - f0 - bnd);
-      else     s += w*((real)2*f1 - (real)2*f0 - (real)2*bnd);
-    }
-  }
-  return s;
+  real tc = w*(fip - (real)2*f0 + fin);
+  real f1 = hip ? fip : fin;
+  real t1;
+  if constexpr (HBB) t1 = w*(f1 - f0 - bnd);
+  else t1 = w*((real)2*f1 - (real)2*f0 - (real)2*bnd);
+  return (hin && hip) ? tc : ((hin || hip) ? t1 : (real)0);
 }
 
 // ==== K2: c stencil -> corrected c, mu, p* =========================================
 // k_pack_lap_c + k_lap_mf + k_mu_axpy + k_grad_cd_mf(c) + k_vel_press_corr
 // (device_engine.cuh:149-196).  exp != nullptr also writes rho, ux, uy, uz for
 // a field download (multiphase_gpu.cuh:704-757).
-template <typename real>
+template <typename real, bool HBB, bool OPEN>
 __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D,
                                                 const int2 *__restrict__ map, unsigned T,
                                                 real *exp, int exp_inst)
@@ -383,22 +439,35 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
   const Inst<real> &d = D[bm.x];
   const unsigned cur = (T ^ (unsigned)d.par0) & 1u;
   bool apply = false; double lambda = 0.0;
-  if (d.mass_on && d.acc[ACC_FLAGS]) lambda = mass_lambda(d, cur, apply);
+  if (d.mass_on && ldg(d.acc, ACC_FLAGS)) lambda = mass_lambda(d, cur, apply);
   const int i = bm.y + (int)threadIdx.x;
   if (i >= d.n) return;
+  constexpr unsigned RB = sizeof(real);
+  const unsigned nsb = (unsigned)d.ns*RB, io = (unsigned)i*RB;
+  const rsrc_t rF = make_rsrc(d.craw, (unsigned)F_COUNT*nsb), rW = make_rsrc(d.words, d.nwords*8u);
   int x, y, z; decode(d, i, x, y, z);
-  int n1[Q]; neighbours1(d, i, x, y, z, n1);
-  const real *craw = d.craw;
-  auto cf = [&](int j) -> real { return corrected<real>(craw[j], apply, lambda); };
-  const real c = cf(i);
-  // grad_c (central) and lap_c with the wetting boundary term
-  real gx, gy, gz; grad_cd<real>(n1, d.hbb, c, cf, gx, gy, gz);
-  real bnd = d.bnd_coeff*c*((real)1 - c);
-  real lapc = laplacian<real>(n1, d.hbb, c, bnd, cf);
+  Nbhd nb; make_nbhd(d, x, y, z, nb);
+  unsigned vo[Q], fl, so;
+  ring1<real, OPEN>(d, rW, nb, io, vo, fl, so);
+  const real c = corrected<real>(bld<real>(rF, io, F_CRAW*nsb), apply, lambda);
+  real cn[Q];
+  #pragma unroll
+  for (int m = 1; m < Q; ++m) cn[m] = corrected<real>(bld<real>(rF, vo[m], F_CRAW*nsb), apply, lambda);
+  real gx = 0, gy = 0, gz = 0, lapc = 0;
+  const real bnd = d.bnd_coeff*c*((real)1 - c);
+  #pragma unroll
+  for (int m = 1; m < Q; ++m) {
+    const int o = OPP[m];
+    const bool hin = (fl >> m) & 1u, hip = (fl >> o) & 1u;
+    bool use; real dd = cd_num<real, HBB>(hin, hip, cn[m], cn[o], use);
+    real t = ((real)3.0*(real)0.5*W<real>(m))*dd;
+    acc_vec<real>(m, use ? t : (real)0, gx, gy, gz);
+    lapc += lap_term<real, HBB>(hin, hip, cn[m], cn[o], c, bnd, (real)3.0*W<real>(m));
+  }
   real c_log = c*((real)1 - c);
   real rho = d.rho1 + d.drho*c;
   real inv = (real)1/((real)(1.0/3.0)*rho);
-  real ux = d.jx[i]*inv, uy = d.jy[i]*inv, uz = d.jz[i]*inv;
+  real ux = bld<real>(rF, io, F_JX*nsb)*inv, uy = bld<real>(rF, io, F_JY*nsb)*inv, uz = bld<real>(rF, io, F_JZ*nsb)*inv;
   real mu = d.four_beta*c_log*((real)0.5 - c);
   mu += -d.kappa*lapc;
   real mor = mu/rho;
@@ -406,14 +475,17 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
   uy = uy + (real)0.5*(d.gy + d.fy*d.ff/rho + mor*gy);
   uz = uz + (real)0.5*(d.gz + d.fz*d.ff/rho + mor*gz);
   real u_dot_drho = d.drho*(ux*gx + uy*gy + uz*gz);
-  real p = d.p0[i];
+  real p = bld<real>(rF, io, F_P0*nsb);
   p += (real)0.5*u_dot_drho*(real)(1.0/3.0);
-  d.c[i] = c; d.mu[i] = mu; d.ps[i] = p;
-  if (exp && bm.x == exp_inst) { int ns = d.ns; exp[i] = rho; exp[ns + i] = ux; exp[2*ns + i] = uy; exp[3*ns + i] = uz; }
+  bst(c, rF, io, F_C*nsb); bst(mu, rF, io, F_MU*nsb); bst(p, rF, io, F_PS*nsb);
+  if (exp && bm.x == exp_inst) {
+    const long ns = d.ns;
+    stg(exp, i, rho); stg(exp, ns + i, ux); stg(exp, 2*ns + i, uy); stg(exp, 3*ns + i, uz);
+  }
 }
 
 // ==== K3: lap(mu) (k_pack_lap_zero + k_lap_mf) ======================================
-template <typename real>
+template <typename real, bool HBB, bool OPEN>
 __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ D,
                                                  const int2 *__restrict__ map, unsigned T)
 {
@@ -422,18 +494,48 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
   const Inst<real> &d = D[bm.x];
   const int i = bm.y + (int)threadIdx.x;
   if (i >= d.n) return;
+  constexpr unsigned RB = sizeof(real);
+  const unsigned nsb = (unsigned)d.ns*RB, io = (unsigned)i*RB;
+  const rsrc_t rF = make_rsrc(d.craw, (unsigned)F_COUNT*nsb), rW = make_rsrc(d.words, d.nwords*8u);
   int x, y, z; decode(d, i, x, y, z);
-  int n1[Q]; neighbours1(d, i, x, y, z, n1);
-  const real *mu = d.mu;
-  auto f = [&](int j) -> real { return mu[j]; };
-  d.lapmu[i] = laplacian<real>(n1, d.hbb, mu[i], (real)0, f);
+  Nbhd nb; make_nbhd(d, x, y, z, nb);
+  unsigned vo[Q], fl, so;
+  ring1<real, OPEN>(d, rW, nb, io, vo, fl, so);
+  real mn[Q];
+  #pragma unroll
+  for (int m = 1; m < Q; ++m) mn[m] = bld<real>(rF, vo[m], F_MU*nsb);
+  const real m0 = bld<real>(rF, io, F_MU*nsb);
+  real s = 0;
+  #pragma unroll
+  for (int m = 1; m < Q; ++m)
+    s += lap_term<real, HBB>((fl >> m) & 1u, (fl >> OPP[m]) & 1u, mn[m], mn[OPP[m]], m0, (real)0, (real)3.0*W<real>(m));
+  bst(s, rF, io, F_LAPMU*nsb);
 }
 
-// ==== K4: fused collision + push streaming ==========================================
+// Push one post-collision population k of the site at byte offset io: into
+// slot k of x+e_k (offset vo), or -- halfway bounce-back -- into slot opp(k)
+// of x, or zero into that slot when x+e_k is outside the box.  The store's
+// scalar offset is min(k, opp k)*ns; the lane part carries the rest.
+template <typename real, bool HBB, bool OPEN>
+__device__ __forceinline__ void push(real v, rsrc_t r, int k, unsigned vo, bool fluid, bool solid,
+                                     unsigned io, unsigned nsb)
+{
+  const int o = OPP[k];
+  const unsigned so = (unsigned)(k < o ? k : o)*nsb;
+  const unsigned bulk_off = (k > o) ? vo + nsb : vo;
+  const unsigned bb_off = (o > k) ? io + nsb : io;
+  const unsigned off = fluid ? bulk_off : bb_off;
+  real val = v;
+  if constexpr (OPEN || !HBB) val = (fluid || (HBB && solid)) ? v : (real)0;
+  (void)solid;
+  bst(val, r, off, so);
+}
+
+// ==== K4: fused collision + push streaming (FP64, reference order) ==================
 // k_collide_fused_bgk (device_engine.cuh:459-545) and k_collide_fused_mrt
 // (:547-651), with the field quantities of the reference's separate kernels
 // (k_moments u, k_grad_cd/bd/md, k_vel_press_corr) recomputed in registers.
-template <typename real, bool MRT>
+template <typename real, bool MRT, bool HBB, bool OPEN>
 __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ D,
                                                  const int2 *__restrict__ map, unsigned T, int stats)
 {
@@ -445,45 +547,67 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
   const bool live = i < d.n;
   Fx s0 = {0,0,0}, s1 = {0,0,0}, s2 = {0,0,0}, s3 = {0,0,0}, s4 = {0,0,0};
   unsigned long long vmax = 0;
-  if (threadIdx.x == 0 && bm.y == 0 && d.mass_on) d.acc[ACC_FLAGS] = 1ull;   // correction owed from now on
+  if (threadIdx.x == 0 && bm.y == 0 && d.mass_on) stg(d.acc, ACC_FLAGS, 1ull);   // correction owed from now on
   if (live) {
-    const int ns = d.ns;
-    const real *gin = cur ? d.g1 : d.g0;
-    real *gout = cur ? d.g0 : d.g1;
+    constexpr unsigned RB = sizeof(real);
+    const unsigned nsb = (unsigned)d.ns*RB, io = (unsigned)i*RB;
+    const rsrc_t rF = make_rsrc(d.craw, (unsigned)F_COUNT*nsb), rW = make_rsrc(d.words, d.nwords*8u);
+    const rsrc_t rGi = make_rsrc(cur ? d.g1 : d.g0, 19u*nsb), rGo = make_rsrc(cur ? d.g0 : d.g1, 19u*nsb);
+    const rsrc_t rH = make_rsrc(d.h, 19u*nsb);
     int x, y, z; decode(d, i, x, y, z);
-    int n1[Q], n2[Q];
-    neighbours1(d, i, x, y, z, n1);
-    neighbours2(d, x, y, z, n2);
-    const real *cF = d.c, *pF = d.ps, *lF = d.lapmu;
-    auto fc = [&](int j) -> real { return cF[j]; };
-    auto fp = [&](int j) -> real { return pF[j]; };
-    const real c = cF[i], p = pF[i], mu = d.mu[i], lmu = lF[i];
+    Nbhd nb; make_nbhd(d, x, y, z, nb);
+    unsigned vo[Q], fl, so, fl2;
+    ring1<real, OPEN>(d, rW, nb, io, vo, fl, so);
+    real cN[Q], pN[Q], c2[Q], p2[Q];
+    {
+      unsigned vo2[Q];
+      ring2<real, OPEN>(d, rW, nb, io, vo2, fl2);
+      #pragma unroll
+      for (int m = 1; m < Q; ++m) {
+        cN[m] = bld<real>(rF, vo[m], F_C*nsb); pN[m] = bld<real>(rF, vo[m], F_PS*nsb);
+        c2[m] = bld<real>(rF, vo2[m], F_C*nsb); p2[m] = bld<real>(rF, vo2[m], F_PS*nsb);
+      }
+    }
+    const real c = bld<real>(rF, io, F_C*nsb), p = bld<real>(rF, io, F_PS*nsb);
+    const real mu = bld<real>(rF, io, F_MU*nsb), lmu = bld<real>(rF, io, F_LAPMU*nsb);
     const real drho = d.drho;
     const real CS2 = (real)(1.0/3.0), ALPHA = (real)3.0, GAMMA_C = (real)1.5, BETA_C = (real)4.5;
     const real rho = d.rho1 + drho*c;
-    real cp = c < (real)0 ? (real)0 : (c > (real)1 ? (real)1 : c);
-    real tau = (real)1/(cp/d.tau0 + ((real)1 - cp)/d.tau1);
+    const real cp = c < (real)0 ? (real)0 : (c > (real)1 ? (real)1 : c);
+    const real tau = (real)1/(cp/d.tau0 + ((real)1 - cp)/d.tau1);
     const real relax = (real)1/(tau + (real)0.5);
-    // own-site post-stream g and its first moment (k_moments order)
     real gk[Q];
     real jx = 0, jy = 0, jz = 0;
     #pragma unroll
-    for (int k = 0; k < Q; ++k) {
-      real gv = gin[k*ns + i]; gk[k] = gv;
-      if (CX[k] > 0) jx += gv; else if (CX[k] < 0) jx -= gv;
-      if (CY[k] > 0) jy += gv; else if (CY[k] < 0) jy -= gv;
-      if (CZ[k] > 0) jz += gv; else if (CZ[k] < 0) jz -= gv;
+    for (int k = 0; k < Q; ++k) { gk[k] = bld<real>(rGi, io, (unsigned)k*nsb); acc_vec<real>(k, gk[k], jx, jy, jz); }
+    // per-direction central and biased values, and the vector gradients in
+    // the reference's direction order
+    real cdc[Q], cdp[Q], bdc[Q], bdp[Q];
+    real e_dcx = 0, e_dcy = 0, e_dcz = 0, e_dpx = 0, e_dpy = 0, e_dpz = 0;
+    real b_cx = 0, b_cy = 0, b_cz = 0, b_px = 0, b_py = 0, b_pz = 0;
+    cdc[0] = cdp[0] = bdc[0] = bdp[0] = 0;
+    #pragma unroll
+    for (int m = 1; m < Q; ++m) {
+      const int o = OPP[m];
+      const bool hin = (fl >> m) & 1u, hip = (fl >> o) & 1u, h2 = (fl2 >> m) & 1u, hp2 = (fl2 >> o) & 1u;
+      bool use;
+      real nc = cd_num<real, HBB>(hin, hip, cN[m], cN[o], use);
+      real np = cd_num<real, HBB>(hin, hip, pN[m], pN[o], use);
+      const real coef = (real)3.0*(real)0.5*W<real>(m), base = (real)3.0*W<real>(m);
+      acc_vec<real>(m, use ? coef*nc : (real)0, e_dcx, e_dcy, e_dcz);
+      acc_vec<real>(m, use ? coef*np : (real)0, e_dpx, e_dpy, e_dpz);
+      cdc[m] = use ? (real)0.5*nc : (real)0;
+      cdp[m] = use ? (real)0.5*np : (real)0;
+      real vc = bd_num<real, HBB>(hin, h2, hip, hp2, cN[m], c2[m], cN[o], c2[o], c);
+      real vp = bd_num<real, HBB>(hin, h2, hip, hp2, pN[m], p2[m], pN[o], p2[o], p);
+      acc_vec<real>(m, base*vc, b_cx, b_cy, b_cz);
+      acc_vec<real>(m, base*vp, b_px, b_py, b_pz);
+      bdc[m] = vc; bdp[m] = vp;
     }
-    // gradients of c and p* (cd, bd, md)
-    real e_dcx, e_dcy, e_dcz, e_dpx, e_dpy, e_dpz, b_cx, b_cy, b_cz, b_px, b_py, b_pz;
-    grad_cd<real>(n1, d.hbb, c, fc, e_dcx, e_dcy, e_dcz);
-    grad_bd<real>(n1, n2, d.hbb, c, fc, b_cx, b_cy, b_cz);
-    grad_cd<real>(n1, d.hbb, p, fp, e_dpx, e_dpy, e_dpz);
-    grad_bd<real>(n1, n2, d.hbb, p, fp, b_px, b_py, b_pz);
     const real f_dcx = (real)0.5*(e_dcx + b_cx), f_dcy = (real)0.5*(e_dcy + b_cy), f_dcz = (real)0.5*(e_dcz + b_cz);
     const real f_dpx = (real)0.5*(e_dpx + b_px), f_dpy = (real)0.5*(e_dpy + b_py), f_dpz = (real)0.5*(e_dpz + b_pz);
     // u* = u + 1/2 (g + f ff/rho + (mu/rho) grad_cd c)   (k_vel_press_corr)
-    real inv = (real)1/(CS2*rho);
+    const real inv = (real)1/(CS2*rho);
     real ux = jx*inv, uy = jy*inv, uz = jz*inv;
     {
       real mor = mu/rho;
@@ -498,7 +622,20 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
       vmax = (unsigned long long)__double_as_longlong(v2);
     }
     const real bfx = rho*d.gx + d.fx*d.ff, bfy = rho*d.gy + d.fy*d.ff, bfz = rho*d.gz + d.fz*d.ff;
-    real *hO = d.h;
+    const real mob = d.mobility;
+    // avg_dir: 1/2 (lmu + lapmu(x+e_k)); a non-site neighbour fetched the own
+    // value, and 1/2 (lmu + lmu) == lmu exactly, the reference's self case
+    auto avg_of = [&](int k) -> real { return (real)0.5*(lmu + bld<real>(rF, vo[k], F_LAPMU*nsb)); };
+    auto dir_dot = [&](int k, real ax, real ay, real az) -> real {
+      real s = 0;
+      if (CX[k] > 0) s += ax; else if (CX[k] < 0) s -= ax;
+      if (CY[k] > 0) s += ay; else if (CY[k] < 0) s -= ay;
+      if (CZ[k] > 0) s += az; else if (CZ[k] < 0) s -= az;
+      return s;
+    };
+    auto pushk = [&](real v, rsrc_t r, int k) {
+      push<real, HBB, OPEN>(v, r, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+    };
     if constexpr (!MRT) {
       const real drcx = drho*e_dcx*CS2, drcy = drho*e_dcy*CS2, drcz = drho*e_dcz*CS2;
       const real e_fgx = mu*e_dcx + bfx, e_fgy = mu*e_dcy + bfy, e_fgz = mu*e_dcz + bfz;
@@ -519,25 +656,11 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
       #pragma unroll
       for (int k = 0; k < Q; ++k) {
         const real w = W<real>(k);
-        real uk = 0;
-        if (CX[k] > 0) uk += ux; else if (CX[k] < 0) uk -= ux;
-        if (CY[k] > 0) uk += uy; else if (CY[k] < 0) uk -= uy;
-        if (CZ[k] > 0) uk += uz; else if (CZ[k] < 0) uk -= uz;
+        real uk = dir_dot(k, ux, uy, uz);
         real u_term = uk*(ALPHA + BETA_C*uk) - GAMMA_C*u_sq;
         real G = w*((real)1 + u_term);
-        real f_proj = 0;
-        if (CX[k] > 0) f_proj += bfx; else if (CX[k] < 0) f_proj -= bfx;
-        if (CY[k] > 0) f_proj += bfy; else if (CY[k] < 0) f_proj -= bfy;
-        if (CZ[k] > 0) f_proj += bfz; else if (CZ[k] < 0) f_proj -= bfz;
-        real cd_c = 0, cd_p = 0, bd_c = 0, bd_p = 0, avgk;
-        if (k > 0) {
-          int in = n1[k], ip = n1[OPP[k]];
-          if (in >= 0 && ip >= 0) { cd_c = (real)0.5*(cF[in] - cF[ip]); cd_p = (real)0.5*(pF[in] - pF[ip]); }
-          else if (d.hbb && ip >= 0) { cd_c = (real)0.5*(c - cF[ip]); cd_p = (real)0.5*(p - pF[ip]); }
-          else if (d.hbb && in >= 0) { cd_c = (real)0.5*(cF[in] - c); cd_p = (real)0.5*(pF[in] - p); }
-          bd_c = bd_val<real>(n1, n2, d.hbb, k, c, fc); bd_p = bd_val<real>(n1, n2, d.hbb, k, p, fp);
-          avgk = in >= 0 ? (real)0.5*(lmu + lF[in]) : lmu;
-        } else avgk = (real)0.5*(lmu + lmu);
+        real f_proj = dir_dot(k, bfx, bfy, bfz);
+        const real cd_c = cdc[k], cd_p = cdp[k];
         real drho_dk = drho*cd_c*CS2;
         real eh = c*G;
         eh -= (real)0.5*G*(cd_c - c*a_over_rho*(cd_p - mu*cd_c - f_proj));
@@ -546,17 +669,14 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
         eg -= (real)0.5*(drho_dk*(G - w) + (mu*cd_c + f_proj)*G);
         eg += (real)0.5*((G - w)*u_drhocd + G*u_fg_e);
         real dGk = G - w;
-        real dc_dk = (real)0.5*(cd_c + bd_c), dp_dk = (real)0.5*(cd_p + bd_p);
+        real dc_dk = (real)0.5*(cd_c + bdc[k]), dp_dk = (real)0.5*(cd_p + bdp[k]);
         real drho_dk2 = drho_cs2*dc_dk;
         real ffk = mu*dc_dk + f_proj;
-        real fh = G*(dc_dk - c_over*(dp_dk - ffk) - u_fh_f + d.mobility*avgk);
+        real fh = G*(dc_dk - c_over*(dp_dk - ffk) - u_fh_f + mob*avg_of(k));
         real fg = (drho_dk2 - u_drm)*dGk + (ffk - u_fg_f)*G;
         real gnew = gk[k] + relax*(eg - gk[k]) + fg;
-        real hnew = eh + fh;
-        int dst = n1[k];
-        if (dst >= 0) { hO[k*ns + dst] = hnew; gout[k*ns + dst] = gnew; }
-        else if (dst == -1 && d.hbb) { hO[OPP[k]*ns + i] = hnew; gout[OPP[k]*ns + i] = gnew; }
-        else { hO[OPP[k]*ns + i] = (real)0; gout[OPP[k]*ns + i] = (real)0; }
+        pushk(eh + fh, rH, k);
+        pushk(gnew, rGo, k);
       }
     } else {
       const real drho_cs2 = drho*CS2, rho_cs2 = rho*CS2, c_over = c/rho_cs2;
@@ -573,80 +693,53 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
       const real u_fh_f = ux*f_fhx + uy*f_fhy + uz*f_fhz;
       const real u_drm = ux*drmx + uy*drmy + uz*drmz;
       const real u_fg_f = ux*f_fgx + uy*f_fgy + uz*f_fgz;
-      real cg[Q], fgs[Q];
+      // relaxation rates (device_engine.cuh:622-625, :640-643)
+      const real s0r = relax; const real taur = (real)1/s0r - (real)0.5;
+      const real s1r = (real)2*taur/((real)2*d.mrt_lambda + taur);
+      // x = S M (eq_g - g), accumulated column by column in the reference's
+      // order as each direction is produced, keeping its (s*M)*y rounding
+      real xx[Q];
+      #pragma unroll
+      for (int m = 0; m < Q; ++m) xx[m] = 0;
+      real fgs[Q];
       #pragma unroll
       for (int k = 0; k < Q; ++k) {
         const real w = W<real>(k);
-        real uk = 0;
-        if (CX[k] > 0) uk += ux; else if (CX[k] < 0) uk -= ux;
-        if (CY[k] > 0) uk += uy; else if (CY[k] < 0) uk -= uy;
-        if (CZ[k] > 0) uk += uz; else if (CZ[k] < 0) uk -= uz;
+        real uk = dir_dot(k, ux, uy, uz);
         real u_term = uk*(ALPHA + BETA_C*uk) - gamma_u_sq;
         real G = w*((real)1 + u_term);
         real w_u_term = G - w;
-        real f_proj = 0;
-        if (CX[k] > 0) f_proj += bfx; else if (CX[k] < 0) f_proj -= bfx;
-        if (CY[k] > 0) f_proj += bfy; else if (CY[k] < 0) f_proj -= bfy;
-        if (CZ[k] > 0) f_proj += bfz; else if (CZ[k] < 0) f_proj -= bfz;
-        real cd_c = 0, cd_p = 0, bd_c = 0, bd_p = 0, avgk;
-        if (k > 0) {
-          int in = n1[k], ip = n1[OPP[k]];
-          if (in >= 0 && ip >= 0) { cd_c = (real)0.5*(cF[in] - cF[ip]); cd_p = (real)0.5*(pF[in] - pF[ip]); }
-          else if (d.hbb && ip >= 0) { cd_c = (real)0.5*(c - cF[ip]); cd_p = (real)0.5*(p - pF[ip]); }
-          else if (d.hbb && in >= 0) { cd_c = (real)0.5*(cF[in] - c); cd_p = (real)0.5*(pF[in] - p); }
-          bd_c = bd_val<real>(n1, n2, d.hbb, k, c, fc); bd_p = bd_val<real>(n1, n2, d.hbb, k, p, fp);
-          avgk = in >= 0 ? (real)0.5*(lmu + lF[in]) : lmu;
-        } else avgk = (real)0.5*(lmu + lmu);
+        real f_proj = dir_dot(k, bfx, bfy, bfz);
+        const real cd_c = cdc[k], cd_p = cdp[k];
         real drho_dk = drho_cs2*cd_c;
         real eh = c*G + (real)0.5*G*(u_term_h - cd_c + c_over*(cd_p - mu*cd_c - f_proj));
         real eg = w*p + rho_cs2*w_u_term + (real)0.5*((u_drho - drho_dk)*w_u_term + (u_force_g + mu*cd_c + f_proj)*G);
         real dGk = G - w;
-        real dc_dk = (real)0.5*(cd_c + bd_c), dp_dk = (real)0.5*(cd_p + bd_p);
+        real dc_dk = (real)0.5*(cd_c + bdc[k]), dp_dk = (real)0.5*(cd_p + bdp[k]);
         real drho_dk2 = drho_cs2*dc_dk;
         real ffk = mu*dc_dk + f_proj;
-        real fh = G*(dc_dk - c_over*(dp_dk - ffk) - u_fh_f + d.mobility*avgk);
-        real fg = (drho_dk2 - u_drm)*dGk + (ffk - u_fg_f)*G;
-        real hnew = eh + fh;
-        int dst = n1[k];
-        if (dst >= 0) hO[k*ns + dst] = hnew;
-        else if (dst == -1 && d.hbb) hO[OPP[k]*ns + i] = hnew;
-        else hO[OPP[k]*ns + i] = (real)0;
-        cg[k] = eg - gk[k]; fgs[k] = fg;
-      }
-      // cg <- M^-1 S M cg (device_engine.cuh:617-648); S from relax and mrt_lambda
-      const real s0r = relax; const real taur = (real)1/s0r - (real)0.5;
-      const real s1r = (real)2*taur/((real)2*d.mrt_lambda + taur);
-      real xx[Q];
-      #pragma unroll
-      for (int m = 0; m < Q; ++m) xx[m] = 0;
-      #pragma unroll
-      for (int nn = 0; nn < Q; ++nn) {
-        real yv = cg[nn];
+        real fh = G*(dc_dk - c_over*(dp_dk - ffk) - u_fh_f + mob*avg_of(k));
+        fgs[k] = (drho_dk2 - u_drm)*dGk + (ffk - u_fg_f)*G;
+        pushk(eh + fh, rH, k);
+        const real yv = eg - gk[k];
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
-          if (MRTTAB.M[m][nn] != 0.0) {
+          if (MRTTAB.M[m][k] != 0.0 && m != 3 && m != 5 && m != 7) {
             const bool odd = (m == 4 || m == 6 || m == 8 || m >= 16);
-            const bool cons = (m == 3 || m == 5 || m == 7);
-            if (!cons) {
-              real sm = odd ? s1r : s0r;
-              xx[m] += sm*(real)MRTTAB.M[m][nn]*yv;
-            }
+            const real sm = odd ? s1r : s0r;
+            xx[m] += sm*(real)MRTTAB.M[m][k]*yv;
           }
         }
       }
       #pragma unroll
-      for (int m = 0; m < Q; ++m) {
+      for (int k = 0; k < Q; ++k) {
         real acc = 0;
         #pragma unroll
-        for (int nn = 0; nn < Q; ++nn) {
-          const bool cons = (nn == 0 || nn == 3 || nn == 5 || nn == 7);
-          if (MRTTAB.MI[m][nn] != 0.0 && !cons) acc += (real)MRTTAB.MI[m][nn]*xx[nn];
+        for (int m = 0; m < Q; ++m) {
+          const bool cons = (m == 0 || m == 3 || m == 5 || m == 7);
+          if (MRTTAB.MI[k][m] != 0.0 && !cons) acc += (real)MRTTAB.MI[k][m]*xx[m];
         }
-        real gnew = gk[m] + acc + fgs[m];
-        int dst = n1[m];
-        if (dst >= 0) gout[m*ns + dst] = gnew;
-        else if (dst == -1 && d.hbb) gout[OPP[m]*ns + i] = gnew;
-        else gout[OPP[m]*ns + i] = (real)0;
+        pushk(gk[k] + acc + fgs[k], rGo, k);
       }
     }
   }
@@ -659,6 +752,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
 }
 
 }  // namespace fm
+
 
 namespace fm {
 
@@ -718,7 +812,8 @@ __global__ void __launch_bounds__(256) k_export_tables(Inst<real> d, int *cdm, i
   if (i >= d.n) return;
   const int n = d.n;
   int x, y, z; decode(d, i, x, y, z);
-  int n1[Q], n2[Q]; neighbours1(d, i, x, y, z, n1); neighbours2(d, x, y, z, n2);
+  Nbhd nb; make_nbhd(d, x, y, z, nb);
+  int n1[Q], n2[Q]; neighbours1(d, nb, i, n1); neighbours2(d, nb, n2);
   for (int m = 1; m < Q; ++m) {
     int in = n1[m], ip = n1[OPP[m]];
     size_t idx = (size_t)(m - 1)*n + i;

@@ -151,6 +151,11 @@ int  felbm_mrt_matrices(double *M, double *Minv);
 /* last kernel timings of one step, in ms: moments, c-stencil, mu-stencil, collide */
 int  felbm_kernel_times(felbm_engine *e, int nsteps, float *ms4);
 
+/* streaming-bandwidth probe: `reps` device-to-device copies of `bytes`;
+ * gbs = (read + write) bytes / s.  Also the calibration kernel for the HBM
+ * byte counters.  No reference counterpart (measurement aid). */
+int  felbm_copy_probe(int device, int64_t bytes, int reps, double *gbs);
+
 const char *felbm_last_error(void);
 const char *felbm_version(void);
 
