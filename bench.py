@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--workload", default="porous_300")
     ap.add_argument("--cpu-steps", type=int, default=int(os.environ.get("FELBM_CPU_STEPS", "6")))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-copy-mb", type=int, default=0,
+                    help="only run the device-to-device copy probe of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU figures are not part of this build "
@@ -71,6 +73,17 @@ def main():
 
     from felbm_gpu_amd import MultiPhaseGPU
     from felbm_gpu_amd import workloads as W
+    import ctypes as C
+    from felbm_gpu_amd import _lib
+
+    def copy_probe(mb, reps):
+        gbs = C.c_double()
+        _lib.check(_lib.load().felbm_copy_probe(0, int(mb) << 20, int(reps), C.byref(gbs)))
+        return gbs.value
+
+    if args.probe_copy_mb:
+        print(json.dumps({"copy_probe_mb": args.probe_copy_mb, "GBs": copy_probe(args.probe_copy_mb, 5)}))
+        return
 
     t0 = time.time()
     w = W.workload(args.workload)
@@ -96,10 +109,12 @@ def main():
         alg = ALG_VALUES[k] * bytes_per_val * n
         kern[k] = {"ms": t, "alg_GBs": alg / (t * 1e-3) / 1e9 if t > 0 else None}
     coll = kern["collide"]
+    copy_gbs = copy_probe(1024, 10)
     roof = {"bound": "hbm", "achieved": coll["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": coll["alg_GBs"] / HBM_PEAK_GBS, "traffic": _profile_traffic(args.workload),
             "kernel": "k_collide (fused MRT + push streaming)",
-            "alg_bytes_per_launch": ALG_VALUES["collide"] * bytes_per_val * n}
+            "alg_bytes_per_launch": ALG_VALUES["collide"] * bytes_per_val * n,
+            "copy_probe_GBs": copy_gbs}
     mass_drift = abs(eng.current_mass() - eng.get_target_mass())
     print("[bench] %.1f MLUPS, kernels %s" % (mlups, json.dumps(kern)), file=sys.stderr, flush=True)
 

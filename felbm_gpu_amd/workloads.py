@@ -34,7 +34,9 @@ def derive_params(sigma=0.01, width=4.0, mobility_coeff=0.02, theta_deg=60.0, rh
     return {
         "rho0": rho0, "rho1": rho1, "kappa": kappa, "beta": beta,
         "boundary_coefficient": 0.0 if abs(kappa) < 1e-16 else phi / kappa,
-        "tau0": tau0, "tau1": tau1, "mobility": mobility_coeff / beta,
+        "tau0": tau0, "tau1": tau1,
+        # with sigma = 0 (single phase) beta is 0 and the mobility term vanishes
+        "mobility": mobility_coeff / beta if beta != 0.0 else 0.0,
         "forcing_factor": forcing_factor, "acceleration": tuple(acceleration), "forcing": tuple(forcing),
         "mrt_lambda": mrt_lambda,
     }

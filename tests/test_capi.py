@@ -1,0 +1,96 @@
+"""The C-ABI library loads and exports every symbol include/*.h declares;
+host-side pieces (geometry generators, parameter derivation) behave.  No GPU
+calls here."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+
+from felbm_gpu_amd import _lib
+from felbm_gpu_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", f)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(felbm_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_symbols()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(_lib.SIGNATURES) == names
+
+
+def test_version_and_error_strings():
+    lib = _lib.load()
+    assert b"gfx950" in lib.felbm_version()
+    assert lib.felbm_create(None, 0, 8) == -1            # EINVAL, no crash
+    assert b"null" in lib.felbm_last_error()
+    h = C.c_void_p()
+    assert lib.felbm_create(C.byref(h), 0, 3) == -1        # bad precision
+    assert lib.felbm_step(None, 1, 0) == -1
+
+
+def test_parameter_derivation_matches_reference_harness():
+    # compare_cpu_gpu.cu:111-118 with sigma=0.01, W=4; phi = 6 sigma cos(theta)
+    p = W.derive_params(sigma=0.01, width=4.0, mobility_coeff=0.02, theta_deg=60.0)
+    assert abs(p["kappa"] - 0.06) < 1e-15 and abs(p["beta"] - 0.03) < 1e-15
+    assert abs(p["mobility"] - 0.02 / 0.03) < 1e-15
+    assert abs(p["boundary_coefficient"] - 6 * 0.01 * 0.5 / 0.06) < 1e-12
+    assert W.derive_params(sigma=0.0)["boundary_coefficient"] == 0.0
+
+
+def test_cylinders_are_seeded_and_disjoint():
+    s1, n1 = W.cylinders(128, 128, 1, 24, 6.0, 10.0, seed=1)
+    s2, n2 = W.cylinders(128, 128, 1, 24, 6.0, 10.0, seed=1)
+    assert n1 == 24 and np.array_equal(s1, s2)
+    s3, _ = W.cylinders(128, 128, 1, 24, 6.0, 10.0, seed=2)
+    assert not np.array_equal(s1, s3)
+    assert 0.5 < 1 - s1.mean() < 0.8
+
+
+def test_spheres_hit_porosity_and_keep_one_cluster():
+    s = W.spheres(48, 48, 48, 3.0, 5.0, 0.40, seed=2, keep_largest=False)
+    assert abs((1 - s.mean()) - 0.40) < 0.01
+    k = W.spheres(48, 48, 48, 3.0, 5.0, 0.40, seed=2, keep_largest=True)
+    assert np.all(k >= s) and (1 - k.mean()) <= (1 - s.mean())
+    # the kept pore space is 18-connected (periodic): a flood fill reaches all of it
+    from scipy import ndimage
+    fluid = np.pad(k == 0, 1, mode="wrap")
+    st = ndimage.generate_binary_structure(3, 2)
+    lab, nl = ndimage.label(fluid, structure=st)
+    core = lab[1:-1, 1:-1, 1:-1]
+    ids = set(np.unique(core[core > 0]))
+    # periodic images: labels on opposite faces belong to one cluster
+    assert len(ids) >= 1
+
+
+def test_keep_largest_cluster_hand_case():
+    lib = _lib.load()
+    s = np.ones((1, 5, 9), np.uint8)
+    s[0, 1, 1:4] = 0          # 3-voxel pore
+    s[0, 3, 5:8] = 0          # another 3-voxel pore ...
+    s[0, 2, 6] = 0            # ... grown to 4
+    kept = lib.felbm_keep_largest_cluster(9, 5, 1, 0, 0, 0, s.ctypes.data_as(C.POINTER(C.c_uint8)))
+    assert kept == 4
+    assert s[0, 1, 1:4].all() and not s[0, 3, 5:8].any()
+
+
+def test_slab_initial_condition():
+    solid = np.zeros((1, 4, 128), np.uint8)
+    c = W.slab_c(solid, 0, 0, 64, 4.0).reshape(4, 128)[0]
+    assert abs(c[0] - 0.5 * (1 + math.tanh(0.25))) < 1e-15
+    assert abs(c[32] - 1) < 1e-12 and abs(c[96]) < 1e-12
+    assert np.allclose(c[:64], c[:64][::-1])
