@@ -61,7 +61,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="porous_300")
-    ap.add_argument("--cpu-steps", type=int, default=int(os.environ.get("FELBM_CPU_STEPS", "6")))
+    ap.add_argument("--cpu-steps", type=int, default=int(os.environ.get("FELBM_CPU_STEPS", "24")))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="only run the device-to-device copy probe of this size (counter calibration)")
