@@ -67,6 +67,7 @@ SIGNATURES = {
     "felbm_step": (_I, [_P, _I, _I]),
     "felbm_step_graph": (_I, [_P, _I, _I]),
     "felbm_set_forcing_factor": (_I, [_P, _I, _D]),
+    "felbm_forcing_factor_at": (_I, [C.c_char_p, _D, C.c_int64, C.POINTER(C.c_double)]),
     "felbm_synchronize": (_I, [_P]),
     "felbm_record_target_mass": (_I, [_P, _I]),
     "felbm_get_target_mass": (_I, [_P, _I, _PD]),

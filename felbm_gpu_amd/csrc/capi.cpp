@@ -1,4 +1,5 @@
 // capi.cpp -- the extern "C" boundary (include/felbm_mi355x.h).
+#include <cmath>
 #include <hip/hip_runtime.h>
 #include <string>
 #include <new>
@@ -55,6 +56,23 @@ int felbm_init_rest(felbm_engine *e, int slot, const double *c, double p0)
 int felbm_step(felbm_engine *e, int nsteps, int stats_last) { ENG(e); return e->impl->step(nsteps, stats_last); }
 int felbm_step_graph(felbm_engine *e, int nsteps, int replays) { ENG(e); return e->impl->step_graph(nsteps, replays); }
 int felbm_set_forcing_factor(felbm_engine *e, int slot, double ff) { ENG(e); return e->impl->set_forcing_factor(slot, ff); }
+
+// forcing_factor_at of the reference driver (src/felbm_gpu_main.cu:587-594):
+// the time-dependence name is matched as a substring, anything else is constant
+int felbm_forcing_factor_at(const char *timedep, double period, int64_t t, double *ff)
+{
+  if (!ff) { fm::set_error("null output pointer"); return FELBM_EINVAL; }
+  const std::string s = timedep ? timedep : "";
+  const double two_pi_t = 2.0 * 3.14159265358979323846 * (double)t;
+  if (s.find("sinusoidal") != std::string::npos) { *ff = std::sin(two_pi_t / period); return FELBM_OK; }
+  if (s.find("square") != std::string::npos) {
+    const double x = std::sin(two_pi_t / period);
+    *ff = (x < 0) ? -1.0 : ((x > 0) ? 1.0 : 0.0);
+    return FELBM_OK;
+  }
+  *ff = 1.0;
+  return FELBM_OK;
+}
 int felbm_synchronize(felbm_engine *e) { ENG(e); return e->impl->synchronize(); }
 int felbm_record_target_mass(felbm_engine *e, int slot) { ENG(e); return e->impl->record_target_mass(slot); }
 int felbm_get_target_mass(felbm_engine *e, int slot, double *m) { ENG(e); if (!m) return FELBM_EINVAL; return e->impl->get_target_mass(slot, m); }

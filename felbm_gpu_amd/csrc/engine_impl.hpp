@@ -260,7 +260,13 @@ public:
     if (int e = check(slot)) return e;
     slots_[slot].prm.forcing_factor = ff;
     slots_[slot].d.ff = (real)ff;
-    dirty_desc_ = true;
+    // descriptors on the device are current: update this one in stream order
+    if (!dirty_ && !dirty_desc_ && d_desc_ && slot_desc_.size() > (size_t)slot && slot_desc_[slot] >= 0) {
+      hipLaunchKernelGGL(k_set_ff<real>, dim3(1), dim3(64), 0, stream_, d_desc_, slot_desc_[slot], (real)ff);
+      FM_LAUNCH();
+    } else {
+      dirty_desc_ = true;
+    }
     return FELBM_OK;
   }
   int synchronize() override { FM_HIP(hipStreamSynchronize(stream_)); return FELBM_OK; }

@@ -847,6 +847,14 @@ __global__ void __launch_bounds__(256) k_export_tables(Inst<real> d, int *cdm, i
   }
 }
 
+// set_forcing_factor as a stream-ordered write into one descriptor, so a
+// per-step forcing schedule needs no host synchronisation
+template <typename real>
+__global__ void k_set_ff(Inst<real> *D, int di, real ff)
+{
+  if (threadIdx.x == 0) stg(&D[di].ff, 0, ff);
+}
+
 template <typename real>
 __global__ void k_zero_words(unsigned long long *p, int count)
 {

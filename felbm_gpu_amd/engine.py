@@ -162,6 +162,15 @@ class MultiPhaseGPU:
     def set_forcing_factor(self, ff):
         check(self.lib.felbm_set_forcing_factor(self.h, self.slot, float(ff)))
 
+    def run(self, t_begin, t_end, timedep="constant", period=1.0):
+        """The driver's stepping loop (src/felbm_gpu_main.cu:701-749 without
+        I/O): for t in [t_begin, t_end) set forcing_factor_at(t), step, apply
+        the mass correction."""
+        for t in range(int(t_begin), int(t_end)):
+            self.set_forcing_factor(forcing_factor_at(timedep, period, t))
+            self.step()
+            self.apply_mass_correction()
+
     def record_target_mass(self):
         check(self.lib.felbm_record_target_mass(self.h, self.slot))
 
@@ -217,6 +226,13 @@ class MultiPhaseGPU:
         if self.slot is not None:
             self.batch.remove(self.slot)
             self.slot = None
+
+
+def forcing_factor_at(timedep, period, t):
+    """forcing_factor_at(t) of the driver (src/felbm_gpu_main.cu:587-594)."""
+    ff = C.c_double()
+    check(_lib.load().felbm_forcing_factor_at(str(timedep).encode(), float(period), int(t), C.byref(ff)))
+    return ff.value
 
 
 def mrt_matrices():

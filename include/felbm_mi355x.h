@@ -95,7 +95,12 @@ int  felbm_init_rest(felbm_engine *e, int slot, const double *c_site, double p0)
  * (:689-700) when correct_op_mass is on, nsteps times, for every slot.
  * stats_last != 0 accumulates the monitoring sums during the final step. */
 int  felbm_step(felbm_engine *e, int nsteps, int stats_last);
-int  felbm_set_forcing_factor(felbm_engine *e, int slot, double ff);   /* P.forcing_factor */
+/* P.forcing_factor (felbm_gpu_main.cu:743); stream-ordered, no host sync */
+int  felbm_set_forcing_factor(felbm_engine *e, int slot, double ff);
+/* forcing_factor_at(t) of the driver (felbm_gpu_main.cu:587-594): timedep
+ * containing "sinusoidal" -> sin(2 pi t / period), "square" -> its sign,
+ * anything else -> 1 */
+int  felbm_forcing_factor_at(const char *timedep, double period, int64_t t, double *ff);
 int  felbm_synchronize(felbm_engine *e);
 /* capture `nsteps` (even) steps in a HIP graph and replay it `replays` times */
 int  felbm_step_graph(felbm_engine *e, int nsteps, int replays);

@@ -1170,6 +1170,19 @@ void orc_delete(orc_engine *E) { if (E) { orc_engine_free(E); free(E); } }
 int orc_n(const orc_engine *E) { return E->n; }
 int orc_inplace_active(const orc_engine *E) { return E->S.stream_inplace; }
 void orc_set_forcing_factor(orc_engine *E, double ff) { E->P.forcing_factor = ff; }
+
+/* forcing_factor_at of the driver loop, src/felbm_gpu_main.cu:587-594.
+ * Pinned by closed forms (quarter-period values, signs, the constant default). */
+double orc_forcing_factor_at(const char *timedep, double period, long long t)
+{
+  const double PI = 3.14159265358979323846;
+  if (timedep && strstr(timedep, "sinusoidal")) return sin(2.0*PI*(double)t/period);
+  if (timedep && strstr(timedep, "square")) {
+    double x = sin(2.0*PI*(double)t/period);
+    return (x < 0) ? -1.0 : ((x > 0) ? 1.0 : 0.0);
+  }
+  return 1.0;
+}
 double orc_get_target_mass(const orc_engine *E) { return E->target_mass; }
 void orc_set_target_mass(orc_engine *E, double m) { E->target_mass = m; }
 void orc_get_coords(const orc_engine *E, int32_t *xyz)

@@ -164,6 +164,7 @@ void orc_delete(orc_engine *E);
 int orc_n(const orc_engine *E);
 int orc_inplace_active(const orc_engine *E);
 void orc_set_forcing_factor(orc_engine *E, double ff);
+double orc_forcing_factor_at(const char *timedep, double period, long long t);
 double orc_get_target_mass(const orc_engine *E);
 void orc_set_target_mass(orc_engine *E, double m);
 void orc_get_coords(const orc_engine *E, int32_t *xyz);

@@ -71,6 +71,7 @@ def load():
         "orc_d3q19": (None, [_PI, _PI, _PI, _PI, _PD]),
         "orc_mrt_matrices": (None, [_PD, _PD]),
         "orc_set_forcing_factor": (None, [C.c_void_p, C.c_double]),
+        "orc_forcing_factor_at": (C.c_double, [C.c_char_p, C.c_double, C.c_longlong]),
         "orc_get_target_mass": (C.c_double, [C.c_void_p]),
         "orc_set_target_mass": (None, [C.c_void_p, C.c_double]),
         "orc_get_coords": (None, [C.c_void_p, _PI]),
@@ -82,6 +83,10 @@ def load():
         f.argtypes = a
     _lib = lib
     return lib
+
+
+def forcing_factor_at(timedep, period, t):
+    return load().orc_forcing_factor_at(timedep.encode(), float(period), int(t))
 
 
 def _d(a):

@@ -94,3 +94,24 @@ def test_slab_initial_condition():
     assert abs(c[0] - 0.5 * (1 + math.tanh(0.25))) < 1e-15
     assert abs(c[32] - 1) < 1e-12 and abs(c[96]) < 1e-12
     assert np.allclose(c[:64], c[:64][::-1])
+
+
+def test_forcing_factor_at_closed_forms():
+    # src/felbm_gpu_main.cu:587-594: sin(2 pi t / T), its sign, else 1
+    from felbm_gpu_amd import forcing_factor_at as ff
+    from oracle.pyoracle import forcing_factor_at as off
+    T = 400.0
+    assert ff("constant", T, 123) == 1.0 and ff("", T, 0) == 1.0
+    assert ff("sinusoidal", T, 0) == 0.0
+    assert abs(ff("sinusoidal", T, 100) - 1.0) < 1e-15
+    assert abs(ff("sinusoidal", T, 300) + 1.0) < 1e-15
+    assert abs(ff("sinusoidal", T, 50) - math.sqrt(0.5)) < 1e-15
+    assert ff("square", T, 0) == 0.0
+    assert ff("square", T, 1) == 1.0 and ff("square", T, 199) == 1.0
+    assert ff("square", T, 201) == -1.0 and ff("square", T, 399) == -1.0
+    # names are matched as substrings, as the reference's driver does
+    assert ff("forcing_sinusoidal_x", T, 100) == ff("sinusoidal", T, 100)
+    # the oracle restatement agrees bit for bit
+    for name in ("sinusoidal", "square", "constant"):
+        for t in (0, 1, 37, 100, 200, 399, 400, 12345):
+            assert ff(name, T, t) == off(name, T, t)

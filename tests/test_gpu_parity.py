@@ -189,6 +189,29 @@ def test_upload_download_roundtrip_and_restart():
     assert np.array_equal(g.download_state()[0], g2.download_state()[0])
 
 
+@pytest.mark.parametrize("timedep", ["sinusoidal", "square"])
+def test_time_dependent_forcing_bitexact(timedep):
+    """The driver loop (src/felbm_gpu_main.cu:701-749): forcing_factor_at(t)
+    set before every step with a body force in x; FP64 bit-identical."""
+    from oracle.pyoracle import forcing_factor_at
+    solid = small_pack(20, 0.55, 5)
+    prm = W.derive_params(theta_deg=60.0, forcing=(2e-5, 0.0, 0.0))
+    o, g = make_pair(solid, prm, True, True)
+    T = 8.0
+    _say("forcing schedule", timedep, "n", g.n)
+    for t in range(3, 15):
+        o.set_forcing_factor(forcing_factor_at(timedep, T, t))
+        o.step()
+        o.apply_mass_correction()
+    g.run(3, 15, timedep, T)
+    ho, go = o.download_state()
+    hg, gg = g.download_state()
+    assert np.array_equal(ho, hg) and np.array_equal(go, gg)
+    fo, fg = o.download(), g.download()
+    for name in ("rho", "c", "ux", "uy", "uz", "p"):
+        assert np.array_equal(fo[name], fg[name]), name
+
+
 def test_fp32_bgk_within_tolerance():
     w = W.workload("2d_cylinders_128")
     o, g = make_pair(w["solid"], w["params"], 0, 1, precision="f32", c0=w["c0"])
