@@ -36,7 +36,10 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
   const bool live = i < d.n;
   Fx s0 = {0,0,0}, s1 = {0,0,0}, s2 = {0,0,0}, s3 = {0,0,0}, s4 = {0,0,0};
   unsigned long long vmax = 0;
-  if (threadIdx.x == 0 && bm.y == 0 && d.mass_on) stg(d.acc, ACC_FLAGS, 1ull);
+  if (threadIdx.x == 0 && bm.y == 0) {
+    if (d.mass_on) stg(d.acc, ACC_FLAGS, 1ull);
+    if (OPEN && d.obc) stg(d.acc, ACC_FLAGS + 1, ldg(d.acc, ACC_FLAGS + 1) + 1ull);   // ++open_step
+  }
   if (live) {
     constexpr unsigned RB = 4;
     const unsigned nsb = (unsigned)d.ns*RB, io = (unsigned)i*RB;
@@ -47,157 +50,189 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
     Nbhd nb; make_nbhd(d, x, y, z, nb);
     unsigned vo[Q], fl, so, fl2;
     ring1<float, OPEN>(d, rW, nb, io, vo, fl, so);
-    float cN[Q], pN[Q], c2[Q], p2[Q];
-    {
-      unsigned vo2[Q];
-      ring2<float, OPEN>(d, rW, nb, io, vo2, fl2);
+    // open-boundary node; a buffer node does not collide and pushes the
+    // equilibrium the reference writes into it (k_open_bnd_buf)
+    int eb = -1;
+    const float *ovp = nullptr;
+    if constexpr (OPEN) {
+      if (d.obc) { eb = ldg(d.ent, i); if (eb >= 0) ovp = d.ov + 8*(long)(eb >> 4); }
+    }
+    const bool nostream = eb >= 0 && (eb & ENT_NOSTREAM);
+    if (nostream) {
+      const float oc = ldg(ovp, 0), orho = ldg(ovp, 1), op = ldg(ovp, 2);
+      const float oux = ldg(ovp, 3), ouy = ldg(ovp, 4), ouz = ldg(ovp, 5);
       #pragma unroll
-      for (int m = 1; m < Q; ++m) {
-        cN[m] = bld<float>(rF, vo[m], F_C*nsb); pN[m] = bld<float>(rF, vo[m], F_PS*nsb);
-        c2[m] = bld<float>(rF, vo2[m], F_C*nsb); p2[m] = bld<float>(rF, vo2[m], F_PS*nsb);
+      for (int k = 0; k < Q; ++k) {
+        float hv, gv;
+        plain_eq<float>(k, oux, ouy, ouz, op, orho, oc, hv, gv);
+        push<float, HBB, OPEN>(hv, rH, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+        push<float, HBB, OPEN>(gv, rGo, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
       }
-    }
-    const float c = bld<float>(rF, io, F_C*nsb), p = bld<float>(rF, io, F_PS*nsb);
-    const float mu = bld<float>(rF, io, F_MU*nsb), lmu = bld<float>(rF, io, F_LAPMU*nsb);
-    const float drho = d.drho;
-    const float CS2 = 1.0f/3.0f;
-    const float rho = d.rho1 + drho*c;
-    const float irho = frcp(rho);
-    const float cp = fminf(fmaxf(c, 0.0f), 1.0f);
-    const float tau = frcp(cp*frcp(d.tau0) + (1.0f - cp)*frcp(d.tau1));
-    const float relax = frcp(tau + 0.5f);
-    // pair numerators and gradients
-    float cdc[Q], cdp[Q], bdc[Q], bdp[Q];
-    float e_dcx = 0, e_dcy = 0, e_dcz = 0, e_dpx = 0, e_dpy = 0, e_dpz = 0;
-    float b_cx = 0, b_cy = 0, b_cz = 0, b_px = 0, b_py = 0, b_pz = 0;
-    cdc[0] = cdp[0] = bdc[0] = bdp[0] = 0.0f;
-    #pragma unroll
-    for (int m = 1; m < Q; m += 2) {
-      const int o = m + 1;
-      const bool hin = (fl >> m) & 1u, hip = (fl >> o) & 1u, h2 = (fl2 >> m) & 1u, hp2 = (fl2 >> o) & 1u;
-      bool use;
-      const float nc = cd_num<float, HBB>(hin, hip, cN[m], cN[o], use);
-      const float np = cd_num<float, HBB>(hin, hip, pN[m], pN[o], use);
-      const float c3w = 3.0f*W<float>(m);
-      acc_vec<float>(m, use ? c3w*nc : 0.0f, e_dcx, e_dcy, e_dcz);
-      acc_vec<float>(m, use ? c3w*np : 0.0f, e_dpx, e_dpy, e_dpz);
-      cdc[m] = use ? 0.5f*nc : 0.0f; cdc[o] = -cdc[m];
-      cdp[m] = use ? 0.5f*np : 0.0f; cdp[o] = -cdp[m];
-      bdc[m] = bd_num<float, HBB>(hin, h2, hip, hp2, cN[m], c2[m], cN[o], c2[o], c);
-      bdp[m] = bd_num<float, HBB>(hin, h2, hip, hp2, pN[m], p2[m], pN[o], p2[o], p);
-      bdc[o] = bd_num<float, HBB>(hip, hp2, hin, h2, cN[o], c2[o], cN[m], c2[m], c);
-      bdp[o] = bd_num<float, HBB>(hip, hp2, hin, h2, pN[o], p2[o], pN[m], p2[m], p);
-      acc_vec<float>(m, c3w*(bdc[m] - bdc[o]), b_cx, b_cy, b_cz);
-      acc_vec<float>(m, c3w*(bdp[m] - bdp[o]), b_px, b_py, b_pz);
-    }
-    float gk[Q];
-    float jx = 0, jy = 0, jz = 0;
-    #pragma unroll
-    for (int k = 0; k < Q; ++k) { gk[k] = bld<float>(rGi, io, (unsigned)k*nsb); acc_vec<float>(k, gk[k], jx, jy, jz); }
-    const float f_dcx = 0.5f*(e_dcx + b_cx), f_dcy = 0.5f*(e_dcy + b_cy), f_dcz = 0.5f*(e_dcz + b_cz);
-    const float f_dpx = 0.5f*(e_dpx + b_px), f_dpy = 0.5f*(e_dpy + b_py), f_dpz = 0.5f*(e_dpz + b_pz);
-    const float inv = 3.0f*irho;
-    const float mor = mu*irho;
-    const float ffr = d.ff*irho;
-    const float ux = jx*inv + 0.5f*(d.gx + d.fx*ffr + mor*e_dcx);
-    const float uy = jy*inv + 0.5f*(d.gy + d.fy*ffr + mor*e_dcy);
-    const float uz = jz*inv + 0.5f*(d.gz + d.fz*ffr + mor*e_dcz);
-    if (stats) {
-      double vx = (double)ux, vy = (double)uy, vz = (double)uz;
-      double v2 = vx*vx + vy*vy + vz*vz;
-      fx_add(s0, vx); fx_add(s1, vy); fx_add(s2, vz); fx_add(s3, v2); fx_add(s4, (double)c);
-      vmax = (unsigned long long)__double_as_longlong(v2);
-    }
-    const float bfx = rho*d.gx + d.fx*d.ff, bfy = rho*d.gy + d.fy*d.ff, bfz = rho*d.gz + d.fz*d.ff;
-    const float drho_cs2 = drho*CS2, rho_cs2 = rho*CS2, c_over = 3.0f*c*irho;
-    const float e_fgx = mu*e_dcx + bfx, e_fgy = mu*e_dcy + bfy, e_fgz = mu*e_dcz + bfz;
-    const float e_fhx = e_dcx - c_over*(e_dpx - e_fgx), e_fhy = e_dcy - c_over*(e_dpy - e_fgy), e_fhz = e_dcz - c_over*(e_dpz - e_fgz);
-    const float f_fgx = mu*f_dcx + bfx, f_fgy = mu*f_dcy + bfy, f_fgz = mu*f_dcz + bfz;
-    const float f_fhx = f_dcx - c_over*(f_dpx - f_fgx), f_fhy = f_dcy - c_over*(f_dpy - f_fgy), f_fhz = f_dcz - c_over*(f_dpz - f_fgz);
-    const float u_sq = ux*ux + uy*uy + uz*uz;
-    const float u_term_h = ux*e_fhx + uy*e_fhy + uz*e_fhz;
-    const float u_force_g = ux*e_fgx + uy*e_fgy + uz*e_fgz;
-    const float u_drho = drho_cs2*(ux*e_dcx + uy*e_dcy + uz*e_dcz);
-    const float u_fh_f = ux*f_fhx + uy*f_fhy + uz*f_fhz;
-    const float u_drm = drho_cs2*(ux*f_dcx + uy*f_dcy + uz*f_dcz);
-    const float u_fg_f = ux*f_fgx + uy*f_fgy + uz*f_fgz;
-    // h_k = G_k (K0 + a2 bd_c + a4 bd_p + a5 e_k.bf + mob avg_k)
-    const float K0 = c + 0.5f*u_term_h - u_fh_f;
-    const float a2 = 0.5f*(1.0f + c_over*mu), a4 = -0.5f*c_over, a5 = 0.5f*c_over;
-    // eq_g = w p + wut (B0 - Bc cd_c) + G (u_force_g/2 +- (fp + mu cd_c)/2)
-    // F_g  = wut (Bc (cd_c + bd_c) - u_drm) + G (mu/2 (cd_c + bd_c) + fp - u_fg_f)
-    const float B0 = rho_cs2 + 0.5f*u_drho, Bc = 0.5f*drho_cs2, hmu = 0.5f*mu, hufg = 0.5f*u_force_g;
-    const float gamma_u_sq = 1.5f*u_sq;
-    const float mob = d.mobility;
-    auto pushk = [&](float v, rsrc_t r, int k) {
-      push<float, HBB, OPEN>(v, r, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
-    };
-    float rr[Q], yv[Q];
-    float relx = 0.0f;
-    if constexpr (!MRT) relx = relax;
-    #pragma unroll
-    for (int k = 0; k < Q; ++k) {
-      const float w = W<float>(k);
-      float uk = 0, fp = 0;
-      if (CX[k] > 0) { uk += ux; fp += bfx; } else if (CX[k] < 0) { uk -= ux; fp -= bfx; }
-      if (CY[k] > 0) { uk += uy; fp += bfy; } else if (CY[k] < 0) { uk -= uy; fp -= bfy; }
-      if (CZ[k] > 0) { uk += uz; fp += bfz; } else if (CZ[k] < 0) { uk -= uz; fp -= bfz; }
-      const float u_term = uk*(3.0f + 4.5f*uk) - gamma_u_sq;
-      const float wut = w*u_term;
-      const float G = w + wut;
-      // 1/2 (lmu + lapmu(x+e_k)); a non-site neighbour fetched lmu itself
-      const float avgk = 0.5f*(lmu + bld<float>(rF, vo[k], F_LAPMU*nsb));
-      const float A = K0 + a2*bdc[k] + a4*bdp[k] + a5*fp + mob*avgk;
-      pushk(G*A, rH, k);
-      const float cdk = cdc[k], dc2 = cdk + bdc[k];
-      // BGK eq_g carries -(mu cd_c + fp)/2 (device_engine.cuh:516), MRT +(...)/2 (:598)
-      const float eg = MRT ? w*p + wut*(B0 - Bc*cdk) + G*(0.5f*fp + hufg + hmu*cdk)
-                           : w*p + wut*(B0 - Bc*cdk) + G*(hufg - 0.5f*fp - hmu*cdk);
-      const float fg = wut*(Bc*dc2 - u_drm) + G*(hmu*dc2 + fp - u_fg_f);
-      if constexpr (!MRT) {
-        pushk(gk[k] + relx*(eg - gk[k]) + fg, rGo, k);
-      } else {
-        yv[k] = eg - gk[k];
-        rr[k] = gk[k] + fg;
+      if (stats) {
+        double vx = (double)oux, vy = (double)ouy, vz = (double)ouz;
+        double v2 = vx*vx + vy*vy + vz*vz;
+        fx_add(s0, vx); fx_add(s1, vy); fx_add(s2, vz); fx_add(s3, v2); fx_add(s4, (double)oc);
+        vmax = (unsigned long long)__double_as_longlong(v2);
       }
-    }
-    if constexpr (MRT) {
-      const float s0r = relax, taur = tau;   // 1/relax - 1/2 == tau
-      const float s1r = 2.0f*taur*frcp(2.0f*d.mrt_lambda + taur);
-      float sp[Q], dp[Q];
-      #pragma unroll
-      for (int m = 1; m < Q; m += 2) { sp[m] = yv[m] + yv[m + 1]; dp[m] = yv[m] - yv[m + 1]; }
-      float xm[Q];
-      #pragma unroll
-      for (int m = 0; m < Q; ++m) {
-        xm[m] = 0.0f;
-        if (CONS_ROW[m]) continue;
-        float a = 0.0f;
-        if (EVEN_ROW[m] && MRTTAB.M[m][0] != 0.0) a += (float)MRTTAB.M[m][0]*yv[0];
-        #pragma unroll
-        for (int k = 1; k < Q; k += 2)
-          if (MRTTAB.M[m][k] != 0.0) a += (float)MRTTAB.M[m][k]*(EVEN_ROW[m] ? sp[k] : dp[k]);
-        const bool odd = (m == 4 || m == 6 || m == 8 || m >= 16);
-        xm[m] = (odd ? s1r : s0r)*((float)(1.0/MRTTAB.N[m])*a);
-      }
+    } else {
+      float cN[Q], pN[Q], c2[Q], p2[Q];
       {
-        float a0 = 0.0f;
+        unsigned vo2[Q];
+        ring2<float, OPEN>(d, rW, nb, io, vo2, fl2);
         #pragma unroll
-        for (int m = 0; m < Q; ++m)
-          if (!CONS_ROW[m] && EVEN_ROW[m] && MRTTAB.M[m][0] != 0.0) a0 += (float)MRTTAB.M[m][0]*xm[m];
-        pushk(rr[0] + a0, rGo, 0);
+        for (int m = 1; m < Q; ++m) {
+          cN[m] = bld<float>(rF, vo[m], F_C*nsb); pN[m] = bld<float>(rF, vo[m], F_PS*nsb);
+          c2[m] = bld<float>(rF, vo2[m], F_C*nsb); p2[m] = bld<float>(rF, vo2[m], F_PS*nsb);
+        }
       }
+      const float c = bld<float>(rF, io, F_C*nsb), p = bld<float>(rF, io, F_PS*nsb);
+      const float mu = bld<float>(rF, io, F_MU*nsb), lmu = bld<float>(rF, io, F_LAPMU*nsb);
+      const float drho = d.drho;
+      const float CS2 = 1.0f/3.0f;
+      const float rho = ovp ? ldg(ovp, 1) : d.rho1 + drho*c;
+      const float irho = frcp(rho);
+      const float crl = ovp ? ldg(ovp, 6) : c;     // relax from the raw c
+      const float cp = fminf(fmaxf(crl, 0.0f), 1.0f);
+      const float tau = frcp(cp*frcp(d.tau0) + (1.0f - cp)*frcp(d.tau1));
+      const float relax = frcp(tau + 0.5f);
+      // pair numerators and gradients
+      float cdc[Q], cdp[Q], bdc[Q], bdp[Q];
+      float e_dcx = 0, e_dcy = 0, e_dcz = 0, e_dpx = 0, e_dpy = 0, e_dpz = 0;
+      float b_cx = 0, b_cy = 0, b_cz = 0, b_px = 0, b_py = 0, b_pz = 0;
+      cdc[0] = cdp[0] = bdc[0] = bdp[0] = 0.0f;
       #pragma unroll
-      for (int k = 1; k < Q; k += 2) {
-        float E = 0.0f, O = 0.0f;
+      for (int m = 1; m < Q; m += 2) {
+        const int o = m + 1;
+        const bool hin = (fl >> m) & 1u, hip = (fl >> o) & 1u, h2 = (fl2 >> m) & 1u, hp2 = (fl2 >> o) & 1u;
+        bool use;
+        const float nc = cd_num<float, HBB>(hin, hip, cN[m], cN[o], use);
+        const float np = cd_num<float, HBB>(hin, hip, pN[m], pN[o], use);
+        const float c3w = 3.0f*W<float>(m);
+        acc_vec<float>(m, use ? c3w*nc : 0.0f, e_dcx, e_dcy, e_dcz);
+        acc_vec<float>(m, use ? c3w*np : 0.0f, e_dpx, e_dpy, e_dpz);
+        cdc[m] = use ? 0.5f*nc : 0.0f; cdc[o] = -cdc[m];
+        cdp[m] = use ? 0.5f*np : 0.0f; cdp[o] = -cdp[m];
+        bdc[m] = bd_num<float, HBB>(hin, h2, hip, hp2, cN[m], c2[m], cN[o], c2[o], c);
+        bdp[m] = bd_num<float, HBB>(hin, h2, hip, hp2, pN[m], p2[m], pN[o], p2[o], p);
+        bdc[o] = bd_num<float, HBB>(hip, hp2, hin, h2, cN[o], c2[o], cN[m], c2[m], c);
+        bdp[o] = bd_num<float, HBB>(hip, hp2, hin, h2, pN[o], p2[o], pN[m], p2[m], p);
+        acc_vec<float>(m, c3w*(bdc[m] - bdc[o]), b_cx, b_cy, b_cz);
+        acc_vec<float>(m, c3w*(bdp[m] - bdp[o]), b_px, b_py, b_pz);
+      }
+      float gk[Q];
+      float jx = 0, jy = 0, jz = 0;
+      #pragma unroll
+      for (int k = 0; k < Q; ++k) { gk[k] = bld<float>(rGi, io, (unsigned)k*nsb); acc_vec<float>(k, gk[k], jx, jy, jz); }
+      const float f_dcx = 0.5f*(e_dcx + b_cx), f_dcy = 0.5f*(e_dcy + b_cy), f_dcz = 0.5f*(e_dcz + b_cz);
+      const float f_dpx = 0.5f*(e_dpx + b_px), f_dpy = 0.5f*(e_dpy + b_py), f_dpz = 0.5f*(e_dpz + b_pz);
+      const float inv = 3.0f*irho;
+      const float mor = mu*irho;
+      const float ffr = d.ff*irho;
+      float ux = jx*inv + 0.5f*(d.gx + d.fx*ffr + mor*e_dcx);
+      float uy = jy*inv + 0.5f*(d.gy + d.fy*ffr + mor*e_dcy);
+      float uz = jz*inv + 0.5f*(d.gz + d.fz*ffr + mor*e_dcz);
+      if (eb >= 0 && (eb & ENT_BCPROC)) { ux = ldg(ovp, 3); uy = ldg(ovp, 4); uz = ldg(ovp, 5); }
+      if (eb >= 0 && (eb & ENT_EQ)) {
+        #pragma unroll
+        for (int k = 0; k < Q; ++k) { float hv; plain_eq<float>(k, ux, uy, uz, p, rho, c, hv, gk[k]); }
+      }
+      if (stats) {
+        double vx = (double)ux, vy = (double)uy, vz = (double)uz;
+        double v2 = vx*vx + vy*vy + vz*vz;
+        fx_add(s0, vx); fx_add(s1, vy); fx_add(s2, vz); fx_add(s3, v2); fx_add(s4, (double)c);
+        vmax = (unsigned long long)__double_as_longlong(v2);
+      }
+      const float bfx = rho*d.gx + d.fx*d.ff, bfy = rho*d.gy + d.fy*d.ff, bfz = rho*d.gz + d.fz*d.ff;
+      const float drho_cs2 = drho*CS2, rho_cs2 = rho*CS2, c_over = 3.0f*c*irho;
+      const float e_fgx = mu*e_dcx + bfx, e_fgy = mu*e_dcy + bfy, e_fgz = mu*e_dcz + bfz;
+      const float e_fhx = e_dcx - c_over*(e_dpx - e_fgx), e_fhy = e_dcy - c_over*(e_dpy - e_fgy), e_fhz = e_dcz - c_over*(e_dpz - e_fgz);
+      const float f_fgx = mu*f_dcx + bfx, f_fgy = mu*f_dcy + bfy, f_fgz = mu*f_dcz + bfz;
+      const float f_fhx = f_dcx - c_over*(f_dpx - f_fgx), f_fhy = f_dcy - c_over*(f_dpy - f_fgy), f_fhz = f_dcz - c_over*(f_dpz - f_fgz);
+      const float u_sq = ux*ux + uy*uy + uz*uz;
+      const float u_term_h = ux*e_fhx + uy*e_fhy + uz*e_fhz;
+      const float u_force_g = ux*e_fgx + uy*e_fgy + uz*e_fgz;
+      const float u_drho = drho_cs2*(ux*e_dcx + uy*e_dcy + uz*e_dcz);
+      const float u_fh_f = ux*f_fhx + uy*f_fhy + uz*f_fhz;
+      const float u_drm = drho_cs2*(ux*f_dcx + uy*f_dcy + uz*f_dcz);
+      const float u_fg_f = ux*f_fgx + uy*f_fgy + uz*f_fgz;
+      // h_k = G_k (K0 + a2 bd_c + a4 bd_p + a5 e_k.bf + mob avg_k)
+      const float K0 = c + 0.5f*u_term_h - u_fh_f;
+      const float a2 = 0.5f*(1.0f + c_over*mu), a4 = -0.5f*c_over, a5 = 0.5f*c_over;
+      // eq_g = w p + wut (B0 - Bc cd_c) + G (u_force_g/2 +- (fp + mu cd_c)/2)
+      // F_g  = wut (Bc (cd_c + bd_c) - u_drm) + G (mu/2 (cd_c + bd_c) + fp - u_fg_f)
+      const float B0 = rho_cs2 + 0.5f*u_drho, Bc = 0.5f*drho_cs2, hmu = 0.5f*mu, hufg = 0.5f*u_force_g;
+      const float gamma_u_sq = 1.5f*u_sq;
+      const float mob = d.mobility;
+      auto pushk = [&](float v, rsrc_t r, int k) {
+        push<float, HBB, OPEN>(v, r, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+      };
+      float rr[Q], yv[Q];
+      float relx = 0.0f;
+      if constexpr (!MRT) relx = relax;
+      #pragma unroll
+      for (int k = 0; k < Q; ++k) {
+        const float w = W<float>(k);
+        float uk = 0, fp = 0;
+        if (CX[k] > 0) { uk += ux; fp += bfx; } else if (CX[k] < 0) { uk -= ux; fp -= bfx; }
+        if (CY[k] > 0) { uk += uy; fp += bfy; } else if (CY[k] < 0) { uk -= uy; fp -= bfy; }
+        if (CZ[k] > 0) { uk += uz; fp += bfz; } else if (CZ[k] < 0) { uk -= uz; fp -= bfz; }
+        const float u_term = uk*(3.0f + 4.5f*uk) - gamma_u_sq;
+        const float wut = w*u_term;
+        const float G = w + wut;
+        // 1/2 (lmu + lapmu(x+e_k)); a non-site neighbour fetched lmu itself
+        const float avgk = 0.5f*(lmu + bld<float>(rF, vo[k], F_LAPMU*nsb));
+        const float A = K0 + a2*bdc[k] + a4*bdp[k] + a5*fp + mob*avgk;
+        pushk(G*A, rH, k);
+        const float cdk = cdc[k], dc2 = cdk + bdc[k];
+        // BGK eq_g carries -(mu cd_c + fp)/2 (device_engine.cuh:516), MRT +(...)/2 (:598)
+        const float eg = MRT ? w*p + wut*(B0 - Bc*cdk) + G*(0.5f*fp + hufg + hmu*cdk)
+                             : w*p + wut*(B0 - Bc*cdk) + G*(hufg - 0.5f*fp - hmu*cdk);
+        const float fg = wut*(Bc*dc2 - u_drm) + G*(hmu*dc2 + fp - u_fg_f);
+        if constexpr (!MRT) {
+          pushk(gk[k] + relx*(eg - gk[k]) + fg, rGo, k);
+        } else {
+          yv[k] = eg - gk[k];
+          rr[k] = gk[k] + fg;
+        }
+      }
+      if constexpr (MRT) {
+        const float s0r = relax, taur = tau;   // 1/relax - 1/2 == tau
+        const float s1r = 2.0f*taur*frcp(2.0f*d.mrt_lambda + taur);
+        float sp[Q], dp[Q];
+        #pragma unroll
+        for (int m = 1; m < Q; m += 2) { sp[m] = yv[m] + yv[m + 1]; dp[m] = yv[m] - yv[m + 1]; }
+        float xm[Q];
         #pragma unroll
         for (int m = 0; m < Q; ++m) {
-          if (CONS_ROW[m] || MRTTAB.M[m][k] == 0.0) continue;
-          if (EVEN_ROW[m]) E += (float)MRTTAB.M[m][k]*xm[m];
-          else O += (float)MRTTAB.M[m][k]*xm[m];
+          xm[m] = 0.0f;
+          if (CONS_ROW[m]) continue;
+          float a = 0.0f;
+          if (EVEN_ROW[m] && MRTTAB.M[m][0] != 0.0) a += (float)MRTTAB.M[m][0]*yv[0];
+          #pragma unroll
+          for (int k = 1; k < Q; k += 2)
+            if (MRTTAB.M[m][k] != 0.0) a += (float)MRTTAB.M[m][k]*(EVEN_ROW[m] ? sp[k] : dp[k]);
+          const bool odd = (m == 4 || m == 6 || m == 8 || m >= 16);
+          xm[m] = (odd ? s1r : s0r)*((float)(1.0/MRTTAB.N[m])*a);
         }
-        pushk(rr[k] + (E + O), rGo, k);
-        pushk(rr[k + 1] + (E - O), rGo, k + 1);
+        {
+          float a0 = 0.0f;
+          #pragma unroll
+          for (int m = 0; m < Q; ++m)
+            if (!CONS_ROW[m] && EVEN_ROW[m] && MRTTAB.M[m][0] != 0.0) a0 += (float)MRTTAB.M[m][0]*xm[m];
+          pushk(rr[0] + a0, rGo, 0);
+        }
+        #pragma unroll
+        for (int k = 1; k < Q; k += 2) {
+          float E = 0.0f, O = 0.0f;
+          #pragma unroll
+          for (int m = 0; m < Q; ++m) {
+            if (CONS_ROW[m] || MRTTAB.M[m][k] == 0.0) continue;
+            if (EVEN_ROW[m]) E += (float)MRTTAB.M[m][k]*xm[m];
+            else O += (float)MRTTAB.M[m][k]*xm[m];
+          }
+          pushk(rr[k] + (E + O), rGo, k);
+          pushk(rr[k + 1] + (E - O), rGo, k + 1);
+        }
       }
     }
   }

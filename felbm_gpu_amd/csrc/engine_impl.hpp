@@ -43,6 +43,11 @@ struct Slot {
   int2 *dmap = nullptr;           // single-instance block map
   int dmap_len = 0;
   int stats_valid = 0;
+  // open boundaries
+  int obc = 0;
+  char *ob_base = nullptr;        // ent [n] + ov [entries*8]
+  std::vector<ObEntry> entries;   // desc filled at rebuild
+  int e0 = 0;                     // first entry in the engine-wide list
 };
 
 template <typename real>
@@ -157,6 +162,7 @@ public:
     if (int e = check(slot)) return e;
     FM_HIP(hipStreamSynchronize(stream_));
     FM_HIP(hipFree(slots_[slot].base));
+    if (slots_[slot].ob_base) FM_HIP(hipFree(slots_[slot].ob_base));
     slots_[slot] = Slot<real>();
     dirty_ = true;
     return rebuild();
@@ -351,10 +357,79 @@ public:
     return FELBM_OK;
   }
 
-  int set_open_bnd(int, const felbm_open_bnd *, const int32_t *, int, const int32_t *, int, const double *,
-                   const int32_t *, const int32_t *, int, int) override {
-    set_error("open boundaries are not implemented in this engine build");
-    return FELBM_EUNSUPPORTED;
+  // MultiPhaseGPU::set_open_bnd (multiphase_gpu.cuh:369-400) with the DevParams
+  // open-BC block (felbm_gpu_main.cu:340-364)
+  int set_open_bnd(int slot, const felbm_open_bnd *ob, const int32_t *inlet, int n_in, const int32_t *outlet,
+                   int n_out, const double *icoord, const int32_t *bv, const int32_t *br, int n_buf,
+                   int copy) override {
+    if (int e = check(slot)) return e;
+    Slot<real> &S = slots_[slot];
+    if (!ob || n_in < 0 || n_out < 0 || n_buf < 0 || (n_in && !inlet) || (n_out && !outlet) ||
+        (n_buf && (!bv || !br))) { set_error("invalid open-boundary lists"); return FELBM_EINVAL; }
+    if (S.mass_on) {
+      set_error("correct_op_mass must be false with open boundaries (mass enters and leaves)");
+      return FELBM_EINVAL;
+    }
+    if (n_buf && !copy) { set_error("buffer nodes are only used with copy_to_buffers"); return FELBM_EINVAL; }
+    if ((ob->inlet_mode == 2 || ob->inlet_mode == 3) && n_in && !icoord) {
+      set_error("split and stripes inlet modes need the inlet coordinates"); return FELBM_EINVAL;
+    }
+    const int n = S.n;
+    std::vector<int> ent((size_t)n, -1), kind((size_t)n, 3), pos((size_t)n, -1);
+    std::vector<ObEntry> E;
+    const bool in_proc = ob->use_in_vel || ob->use_in_prs || ob->use_in_fluid;
+    const bool out_proc = ob->use_out_prs || ob->use_out_fluid;
+    auto add = [&](int site, int ref, int rkind, double rc, int bits) -> bool {
+      if (site < 0 || site >= n || ref < 0 || ref >= n || ent[site] >= 0) return false;
+      ObEntry e{}; e.desc = -1; e.site = site; e.ref = ref; e.rkind = rkind; e.eid = (int)E.size(); e.rcoord = rc;
+      ent[site] = (e.eid << 4) | bits;
+      E.push_back(e);
+      return true;
+    };
+    for (int q = 0; q < n_in; ++q) {
+      if (!add(inlet[q], inlet[q], 0, icoord ? icoord[q] : 0.0, (in_proc ? ENT_BCPROC : 0) | (copy ? 0 : ENT_EQ))) {
+        set_error("inlet node out of range or listed twice"); return FELBM_EINVAL;
+      }
+      kind[inlet[q]] = 0; pos[inlet[q]] = q;
+    }
+    for (int q = 0; q < n_out; ++q) {
+      if (!add(outlet[q], outlet[q], 1, 0.0, (out_proc ? ENT_BCPROC : 0) | (copy ? 0 : ENT_EQ))) {
+        set_error("outlet node out of range or listed twice"); return FELBM_EINVAL;
+      }
+      kind[outlet[q]] = 1;
+    }
+    for (int q = 0; q < n_buf; ++q) {
+      const int r = br[q];
+      if (r < 0 || r >= n) { set_error("buffer reference out of range"); return FELBM_EINVAL; }
+      const double rc = (kind[r] == 0 && icoord) ? icoord[pos[r]] : 0.0;
+      if (!add(bv[q], r, kind[r], rc, ENT_BCPROC | ENT_NOSTREAM)) {
+        set_error("buffer node out of range or listed twice"); return FELBM_EINVAL;
+      }
+    }
+    FM_HIP(hipStreamSynchronize(stream_));
+    if (S.ob_base) { FM_HIP(hipFree(S.ob_base)); S.ob_base = nullptr; }
+    const size_t ent_bytes = align_up((size_t)n * sizeof(int), 256);
+    const size_t bytes = ent_bytes + std::max<size_t>(1, E.size()) * 8 * sizeof(real);
+    if (hipMalloc((void **)&S.ob_base, bytes) != hipSuccess) { set_error("device allocation failed"); return FELBM_ENOMEM; }
+    FM_HIP(hipMemcpyAsync(S.ob_base, ent.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, stream_));
+    FM_HIP(hipMemsetAsync(S.ob_base + ent_bytes, 0, bytes - ent_bytes, stream_));
+    Inst<real> &d = S.d;
+    d.ent = (const int *)S.ob_base; d.ov = (real *)(S.ob_base + ent_bytes);
+    d.obc = 1;
+    d.use_in_vel = ob->use_in_vel; d.use_in_prs = ob->use_in_prs; d.use_in_fluid = ob->use_in_fluid;
+    d.use_out_prs = ob->use_out_prs; d.use_out_fluid = ob->use_out_fluid; d.inlet_mode = ob->inlet_mode;
+    for (int a = 0; a < 3; ++a) d.u_in[a] = ob->u_in[a];
+    d.p_in = ob->p_in; d.p_out = ob->p_out; d.c_out_fixed = ob->c_out_fixed; d.rho_out_fixed = ob->rho_out_fixed;
+    d.inlet_c_a = ob->inlet_c_a; d.inlet_period = ob->inlet_period; d.inlet_duty = ob->inlet_duty;
+    d.inlet_ramp = ob->inlet_ramp; d.split_pos = ob->split_pos;
+    // the injection schedule starts at step 0 (open_step, multiphase_gpu.cuh:445)
+    static const unsigned long long zero = 0;
+    FM_HIP(hipMemcpyAsync(d.acc + ACC_FLAGS + 1, &zero, sizeof(zero), hipMemcpyHostToDevice, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    S.obc = 1;
+    S.entries = std::move(E);
+    dirty_ = true;
+    return FELBM_OK;
   }
 
   int export_tables(int slot, int32_t **tabs) override {
@@ -408,7 +483,9 @@ private:
   int nvar_[8] = {};
   int map_cap_ = 0;
   int nmap_ = 0;
-  static int variant(const Slot<real> &s) { return (s.mrt ? 1 : 0) | (s.hbb ? 2 : 0) | (s.open ? 4 : 0); }
+  static int variant(const Slot<real> &s) { return (s.mrt ? 1 : 0) | (s.hbb ? 2 : 0) | ((s.open || s.obc) ? 4 : 0); }
+  ObEntry *d_ent_all_ = nullptr;    // open-boundary nodes of every slot
+  int n_ent_all_ = 0, ent_cap_ = 0;
   bool dirty_ = false, dirty_desc_ = false;
   unsigned T_ = 0;
   unsigned map_ver_ = 0;
@@ -440,6 +517,8 @@ private:
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     for (auto &s : slots_) if (s.used && s.base) (void)hipFree(s.base);
+    for (auto &s : slots_) if (s.used && s.ob_base) (void)hipFree(s.ob_base);
+    if (d_ent_all_) (void)hipFree(d_ent_all_);
     slots_.clear();
     if (d_desc_) (void)hipFree(d_desc_);
     if (d_map_) (void)hipFree(d_map_);
@@ -490,6 +569,22 @@ private:
         if (nvar_[k]) FM_HIP(hipMemcpyAsync(p, var[k].data(), sizeof(int2) * nvar_[k], hipMemcpyHostToDevice, stream_));
         p += nvar_[k];
       }
+    }
+    // open-boundary nodes of every slot, one list for one launch per step
+    {
+      std::vector<ObEntry> all_e;
+      for (size_t q = 0; q < slots_.size(); ++q) {
+        if (!slots_[q].used || !slots_[q].obc) continue;
+        slots_[q].e0 = (int)all_e.size();
+        for (ObEntry e : slots_[q].entries) { e.desc = slot_desc_[q]; all_e.push_back(e); }
+      }
+      n_ent_all_ = (int)all_e.size();
+      if (n_ent_all_ > ent_cap_) {
+        if (d_ent_all_) FM_HIP(hipFree(d_ent_all_));
+        ent_cap_ = std::max(n_ent_all_, 2 * ent_cap_);
+        FM_HIP(hipMalloc((void **)&d_ent_all_, sizeof(ObEntry) * ent_cap_));
+      }
+      if (n_ent_all_) FM_HIP(hipMemcpyAsync(d_ent_all_, all_e.data(), sizeof(ObEntry) * n_ent_all_, hipMemcpyHostToDevice, stream_));
     }
     // single-instance maps (field downloads) carry the new descriptor indices
     std::vector<std::vector<int2>> single(slots_.size());
@@ -556,6 +651,10 @@ private:
     dim3 blk(256);
     if (timed) FM_HIP(hipEventRecord(ev_[0], stream_));
     hipLaunchKernelGGL(k_moments<real>, dim3(nmap_), blk, 0, stream_, d_desc_, d_map_, T_); FM_LAUNCH();
+    if (n_ent_all_) {
+      hipLaunchKernelGGL(k_open_fields<real>, dim3((n_ent_all_ + 255) / 256), blk, 0, stream_, d_desc_, d_ent_all_, n_ent_all_, T_);
+      FM_LAUNCH();
+    }
     if (timed) FM_HIP(hipEventRecord(ev_[1], stream_));
     for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_key(k, 2, dim3(nvar_[k]), d_var_[k], stats); FM_LAUNCH(); }
     if (timed) FM_HIP(hipEventRecord(ev_[2], stream_));
@@ -613,6 +712,11 @@ private:
     const size_t ns = (size_t)S.ns;
     dim3 blk(256), grid(S.dmap_len);
     hipLaunchKernelGGL(k_moments<real>, grid, blk, 0, stream_, d_desc_, S.dmap, T_); FM_LAUNCH();
+    if (S.obc && !S.entries.empty()) {
+      const int ne = (int)S.entries.size();
+      hipLaunchKernelGGL(k_open_fields<real>, dim3((ne + 255) / 256), blk, 0, stream_, d_desc_, d_ent_all_ + S.e0, ne, T_);
+      FM_LAUNCH();
+    }
     launch_key(variant(S), 2, grid, S.dmap, false, S.d.craw + export_off_ * ns, di); FM_LAUNCH();
     // the step's own moments pass must find this parity's partials at zero
     unsigned cur = cur_of(S);

@@ -90,12 +90,29 @@ struct Inst {
   real rho0, rho1, drho, kappa, four_beta, bnd_coeff, tau0, tau1, mobility, mrt_lambda;
   real gx, gy, gz, fx, fy, fz, ff;
   double M0;
+  // open boundaries (the DevParams open-BC block, device_engine.cuh:45-58)
+  int obc;                     // inlet/outlet lists present
+  const int *ent;              // [n]: -1, or (entry << 4) | ENT_* bits
+  real *ov;                    // [entries][8]: c, rho, p, ux, uy, uz, own raw c
+  int use_in_vel, use_in_prs, use_in_fluid, use_out_prs, use_out_fluid, inlet_mode;
+  double u_in[3], p_in, p_out, c_out_fixed, rho_out_fixed;
+  double inlet_c_a, inlet_period, inlet_duty, inlet_ramp, split_pos;
 };
+
+// site classes of the open boundary
+constexpr int ENT_BCPROC = 1;     // fields imposed: no velocity/pressure correction
+constexpr int ENT_EQ = 2;         // g replaced by the equilibrium of the site's fields
+constexpr int ENT_NOSTREAM = 4;   // buffer node: no collision, pushes its equilibrium
+
+// one open-boundary node: `site` takes the imposed fields of `ref` (itself for
+// inlet/outlet nodes, the mirrored vertex for a buffer node); rkind is the
+// ref's class (0 inlet, 1 outlet, 3 neither), rcoord its split-axis coordinate
+struct ObEntry { int desc, site, ref, rkind, eid, pad; double rcoord; };
 
 // accumulator layout (unsigned long long, NSLOT slots per quantity)
 constexpr int ACC_MASS = 0;                 // [parity][5][NSLOT]: M.hi M.lo W.hi W.lo bad
 constexpr int ACC_STATS = 10*NSLOT;         // [12][NSLOT]: 5 sums x (hi,lo), max bits, bad
-constexpr int ACC_FLAGS = 22*NSLOT;         // [0] pending correction
+constexpr int ACC_FLAGS = 22*NSLOT;         // [0] pending correction, [1] open-boundary step
 constexpr int ACC_MISC = 22*NSLOT + 8;      // [5][NSLOT] single-instance reductions
 constexpr int ACC_WORDS = 27*NSLOT + 8;
 
@@ -384,6 +401,108 @@ __global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ 
   }
 }
 
+// ==== open boundaries: imposed fields (k_open_bnd_fields + k_open_bnd_buf_fields,
+// device_engine.cuh:738-759, :807-815) ================================================
+
+// injection schedule (d_inlet_gate, device_engine.cuh:699-736), evaluated in FP64
+template <typename real>
+__device__ inline double inlet_gate(const Inst<real> &d, unsigned step, double coord)
+{
+  const double w = d.inlet_ramp;
+  if (d.inlet_mode == 1) {
+    const double Per = d.inlet_period;
+    if (Per <= 0.0) return 1.0;
+    // the reference floors the period count in single precision
+    const double tau = (double)step - Per*(double)floorf((float)((double)step/Per));
+    const double lo = 0.5*Per - 0.5*d.inlet_duty*Per, hi = 0.5*Per + 0.5*d.inlet_duty*Per;
+    if (w <= 0.0) return (tau >= lo && tau < hi) ? 1.0 : 0.0;
+    return 0.5*(tanh(2.0*(tau - lo)/w) - tanh(2.0*(tau - hi)/w));
+  }
+  if (d.inlet_mode == 2) {
+    const double x = coord - d.split_pos;
+    if (w <= 0.0) return (x < 0.0) ? 1.0 : 0.0;
+    return 0.5*(1.0 - tanh(2.0*x/w));
+  }
+  if (d.inlet_mode == 3) {
+    const double Per = d.inlet_period;
+    if (Per <= 0.0) return 1.0;
+    double t = coord - d.split_pos;
+    t = t - Per*floor(t/Per);
+    const double lo = 0.5*Per - 0.5*d.inlet_duty*Per, hi = 0.5*Per + 0.5*d.inlet_duty*Per;
+    if (w <= 0.0) return (t >= lo && t < hi) ? 1.0 : 0.0;
+    return 0.5*(tanh(2.0*(t - lo)/w) - tanh(2.0*(t - hi)/w));
+  }
+  return 1.0;
+}
+
+// One thread per open-boundary node, after the moments pass.  The fields the
+// reference imposes (c, rho, p, u) are computed from the mirrored vertex's
+// distributions, stored in ov, and c is written into the c_raw field so every
+// stencil of the step reads the imposed value; the node's own raw c is kept for
+// its bulk chemical potential and relaxation rate, which the reference leaves
+// as computed from the distributions.
+template <typename real>
+__global__ void __launch_bounds__(256) k_open_fields(const Inst<real> *__restrict__ D,
+                                                     const ObEntry *__restrict__ E, int ne, unsigned T)
+{
+  const int j = blockIdx.x*blockDim.x + (int)threadIdx.x;
+  if (j >= ne) return;
+  const ObEntry e = E[j];
+  const Inst<real> &d = D[e.desc];
+  const unsigned cur = (T ^ (unsigned)d.par0) & 1u;
+  const real *g = cur ? d.g1 : d.g0;
+  const long ns = d.ns;
+  // k_moments at the mirrored vertex (device_engine.cuh:116-135)
+  real c = 0, p = 0, ux = 0, uy = 0, uz = 0;
+  #pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    const real hv = ldg(d.h, k*ns + e.ref), gv = ldg(g, k*ns + e.ref);
+    c += hv; p += gv;
+    acc_vec<real>(k, gv, ux, uy, uz);
+  }
+  const real cr = c;
+  real rho = d.rho1 + d.drho*c;
+  const real inv = (real)1/((real)(1.0/3.0)*rho);
+  ux *= inv; uy *= inv; uz *= inv;
+  const unsigned step = (unsigned)ldg(d.acc, ACC_FLAGS + 1);
+  if (e.rkind == 0) {
+    if (d.use_in_vel) { ux = (real)d.u_in[0]; uy = (real)d.u_in[1]; uz = (real)d.u_in[2]; }
+    if (d.use_in_prs) p = (real)d.p_in;
+    if (d.use_in_fluid) {
+      const double gate = inlet_gate(d, step, e.rcoord);
+      const double cv = (1.0 - d.inlet_c_a) + (2.0*d.inlet_c_a - 1.0)*gate;
+      c = (real)cv;
+      rho = (real)(cv*(double)d.rho0 + (1.0 - cv)*(double)d.rho1);
+    }
+  } else if (e.rkind == 1) {
+    if (d.use_out_prs) p = (real)d.p_out;
+    if (d.use_out_fluid) { c = (real)d.c_out_fixed; rho = (real)d.rho_out_fixed; }
+  }
+  real craw_own = cr;
+  if (e.site != e.ref) {
+    craw_own = 0;
+    #pragma unroll
+    for (int k = 0; k < Q; ++k) craw_own += ldg(d.h, k*ns + e.site);
+  }
+  real *o = d.ov + 8*(long)e.eid;
+  stg(o, 0, c); stg(o, 1, rho); stg(o, 2, p); stg(o, 3, ux); stg(o, 4, uy); stg(o, 5, uz); stg(o, 6, craw_own);
+  stg(d.craw, (long)e.site, c);
+}
+
+// equilibrium the open boundary writes (k_open_bnd / k_open_bnd_buf,
+// device_engine.cuh:772-779, :838-845)
+template <typename real>
+__device__ __forceinline__ void plain_eq(int k, real ux, real uy, real uz, real pp, real rho, real c,
+                                         real &h, real &g)
+{
+  const real w = (real)WD[k];
+  const real u_sq = ux*ux + uy*uy + uz*uz;
+  const real uk = ux*(real)CX[k] + uy*(real)CY[k] + uz*(real)CZ[k];
+  const real u_term = uk*((real)3.0 + (real)4.5*uk) - (real)1.5*u_sq;
+  h = c*w*((real)1 + u_term);
+  g = w*(pp + rho*(real)(1.0/3.0)*u_term);
+}
+
 // ---- per-direction stencil values ---------------------------------------------------
 // Neighbour values are fetched at the own site when the neighbour is not a site,
 // which makes the halfway-bounce-back central difference f(x+e)-f(x-e) already
@@ -464,19 +583,32 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
     acc_vec<real>(m, use ? t : (real)0, gx, gy, gz);
     lapc += lap_term<real, HBB>(hin, hip, cn[m], cn[o], c, bnd, (real)3.0*W<real>(m));
   }
-  real c_log = c*((real)1 - c);
-  real rho = d.rho1 + d.drho*c;
+  // open-boundary node: imposed rho (and u, p when the fields are imposed);
+  // mu_bulk from the node's own raw c, as the reference's moments pass has it
+  int eb = -1;
+  const real *ovp = nullptr;
+  if constexpr (OPEN) {
+    if (d.obc) { eb = ldg(d.ent, i); if (eb >= 0) ovp = d.ov + 8*(long)(eb >> 4); }
+  }
+  const real cm = ovp ? ldg(ovp, 6) : c;
+  real c_log = cm*((real)1 - cm);
+  real rho = ovp ? ldg(ovp, 1) : d.rho1 + d.drho*c;
   real inv = (real)1/((real)(1.0/3.0)*rho);
   real ux = bld<real>(rF, io, F_JX*nsb)*inv, uy = bld<real>(rF, io, F_JY*nsb)*inv, uz = bld<real>(rF, io, F_JZ*nsb)*inv;
-  real mu = d.four_beta*c_log*((real)0.5 - c);
+  real mu = d.four_beta*c_log*((real)0.5 - cm);
   mu += -d.kappa*lapc;
-  real mor = mu/rho;
-  ux = ux + (real)0.5*(d.gx + d.fx*d.ff/rho + mor*gx);
-  uy = uy + (real)0.5*(d.gy + d.fy*d.ff/rho + mor*gy);
-  uz = uz + (real)0.5*(d.gz + d.fz*d.ff/rho + mor*gz);
-  real u_dot_drho = d.drho*(ux*gx + uy*gy + uz*gz);
   real p = bld<real>(rF, io, F_P0*nsb);
-  p += (real)0.5*u_dot_drho*(real)(1.0/3.0);
+  if (eb >= 0 && (eb & ENT_BCPROC)) {
+    // k_vel_press_corr skips nodes whose fields the boundary imposed
+    ux = ldg(ovp, 3); uy = ldg(ovp, 4); uz = ldg(ovp, 5); p = ldg(ovp, 2);
+  } else {
+    real mor = mu/rho;
+    ux = ux + (real)0.5*(d.gx + d.fx*d.ff/rho + mor*gx);
+    uy = uy + (real)0.5*(d.gy + d.fy*d.ff/rho + mor*gy);
+    uz = uz + (real)0.5*(d.gz + d.fz*d.ff/rho + mor*gz);
+    real u_dot_drho = d.drho*(ux*gx + uy*gy + uz*gz);
+    p += (real)0.5*u_dot_drho*(real)(1.0/3.0);
+  }
   bst(c, rF, io, F_C*nsb); bst(mu, rF, io, F_MU*nsb); bst(p, rF, io, F_PS*nsb);
   if (exp && bm.x == exp_inst) {
     const long ns = d.ns;
@@ -547,7 +679,10 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
   const bool live = i < d.n;
   Fx s0 = {0,0,0}, s1 = {0,0,0}, s2 = {0,0,0}, s3 = {0,0,0}, s4 = {0,0,0};
   unsigned long long vmax = 0;
-  if (threadIdx.x == 0 && bm.y == 0 && d.mass_on) stg(d.acc, ACC_FLAGS, 1ull);   // correction owed from now on
+  if (threadIdx.x == 0 && bm.y == 0) {
+    if (d.mass_on) stg(d.acc, ACC_FLAGS, 1ull);                   // correction owed from now on
+    if (OPEN && d.obc) stg(d.acc, ACC_FLAGS + 1, ldg(d.acc, ACC_FLAGS + 1) + 1ull);   // ++open_step
+  }
   if (live) {
     constexpr unsigned RB = sizeof(real);
     const unsigned nsb = (unsigned)d.ns*RB, io = (unsigned)i*RB;
@@ -558,188 +693,222 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
     Nbhd nb; make_nbhd(d, x, y, z, nb);
     unsigned vo[Q], fl, so, fl2;
     ring1<real, OPEN>(d, rW, nb, io, vo, fl, so);
-    real cN[Q], pN[Q], c2[Q], p2[Q];
-    {
-      unsigned vo2[Q];
-      ring2<real, OPEN>(d, rW, nb, io, vo2, fl2);
-      #pragma unroll
-      for (int m = 1; m < Q; ++m) {
-        cN[m] = bld<real>(rF, vo[m], F_C*nsb); pN[m] = bld<real>(rF, vo[m], F_PS*nsb);
-        c2[m] = bld<real>(rF, vo2[m], F_C*nsb); p2[m] = bld<real>(rF, vo2[m], F_PS*nsb);
-      }
+    // open-boundary node (ENT_* bits); a buffer node does not collide and
+    // pushes the equilibrium the reference writes into it (k_open_bnd_buf)
+    int eb = -1;
+    const real *ovp = nullptr;
+    if constexpr (OPEN) {
+      if (d.obc) { eb = ldg(d.ent, i); if (eb >= 0) ovp = d.ov + 8*(long)(eb >> 4); }
     }
-    const real c = bld<real>(rF, io, F_C*nsb), p = bld<real>(rF, io, F_PS*nsb);
-    const real mu = bld<real>(rF, io, F_MU*nsb), lmu = bld<real>(rF, io, F_LAPMU*nsb);
-    const real drho = d.drho;
-    const real CS2 = (real)(1.0/3.0), ALPHA = (real)3.0, GAMMA_C = (real)1.5, BETA_C = (real)4.5;
-    const real rho = d.rho1 + drho*c;
-    const real cp = c < (real)0 ? (real)0 : (c > (real)1 ? (real)1 : c);
-    const real tau = (real)1/(cp/d.tau0 + ((real)1 - cp)/d.tau1);
-    const real relax = (real)1/(tau + (real)0.5);
-    real gk[Q];
-    real jx = 0, jy = 0, jz = 0;
-    #pragma unroll
-    for (int k = 0; k < Q; ++k) { gk[k] = bld<real>(rGi, io, (unsigned)k*nsb); acc_vec<real>(k, gk[k], jx, jy, jz); }
-    // per-direction central and biased values, and the vector gradients in
-    // the reference's direction order
-    real cdc[Q], cdp[Q], bdc[Q], bdp[Q];
-    real e_dcx = 0, e_dcy = 0, e_dcz = 0, e_dpx = 0, e_dpy = 0, e_dpz = 0;
-    real b_cx = 0, b_cy = 0, b_cz = 0, b_px = 0, b_py = 0, b_pz = 0;
-    cdc[0] = cdp[0] = bdc[0] = bdp[0] = 0;
-    #pragma unroll
-    for (int m = 1; m < Q; ++m) {
-      const int o = OPP[m];
-      const bool hin = (fl >> m) & 1u, hip = (fl >> o) & 1u, h2 = (fl2 >> m) & 1u, hp2 = (fl2 >> o) & 1u;
-      bool use;
-      real nc = cd_num<real, HBB>(hin, hip, cN[m], cN[o], use);
-      real np = cd_num<real, HBB>(hin, hip, pN[m], pN[o], use);
-      const real coef = (real)3.0*(real)0.5*W<real>(m), base = (real)3.0*W<real>(m);
-      acc_vec<real>(m, use ? coef*nc : (real)0, e_dcx, e_dcy, e_dcz);
-      acc_vec<real>(m, use ? coef*np : (real)0, e_dpx, e_dpy, e_dpz);
-      cdc[m] = use ? (real)0.5*nc : (real)0;
-      cdp[m] = use ? (real)0.5*np : (real)0;
-      real vc = bd_num<real, HBB>(hin, h2, hip, hp2, cN[m], c2[m], cN[o], c2[o], c);
-      real vp = bd_num<real, HBB>(hin, h2, hip, hp2, pN[m], p2[m], pN[o], p2[o], p);
-      acc_vec<real>(m, base*vc, b_cx, b_cy, b_cz);
-      acc_vec<real>(m, base*vp, b_px, b_py, b_pz);
-      bdc[m] = vc; bdp[m] = vp;
-    }
-    const real f_dcx = (real)0.5*(e_dcx + b_cx), f_dcy = (real)0.5*(e_dcy + b_cy), f_dcz = (real)0.5*(e_dcz + b_cz);
-    const real f_dpx = (real)0.5*(e_dpx + b_px), f_dpy = (real)0.5*(e_dpy + b_py), f_dpz = (real)0.5*(e_dpz + b_pz);
-    // u* = u + 1/2 (g + f ff/rho + (mu/rho) grad_cd c)   (k_vel_press_corr)
-    const real inv = (real)1/(CS2*rho);
-    real ux = jx*inv, uy = jy*inv, uz = jz*inv;
-    {
-      real mor = mu/rho;
-      ux = ux + (real)0.5*(d.gx + d.fx*d.ff/rho + mor*e_dcx);
-      uy = uy + (real)0.5*(d.gy + d.fy*d.ff/rho + mor*e_dcy);
-      uz = uz + (real)0.5*(d.gz + d.fz*d.ff/rho + mor*e_dcz);
-    }
-    if (stats) {                  // k_flow_stats, device_engine.cuh:660-683
-      double vx = (double)ux, vy = (double)uy, vz = (double)uz;
-      double v2 = vx*vx + vy*vy + vz*vz;
-      fx_add(s0, vx); fx_add(s1, vy); fx_add(s2, vz); fx_add(s3, v2); fx_add(s4, (double)c);
-      vmax = (unsigned long long)__double_as_longlong(v2);
-    }
-    const real bfx = rho*d.gx + d.fx*d.ff, bfy = rho*d.gy + d.fy*d.ff, bfz = rho*d.gz + d.fz*d.ff;
-    const real mob = d.mobility;
-    // avg_dir: 1/2 (lmu + lapmu(x+e_k)); a non-site neighbour fetched the own
-    // value, and 1/2 (lmu + lmu) == lmu exactly, the reference's self case
-    auto avg_of = [&](int k) -> real { return (real)0.5*(lmu + bld<real>(rF, vo[k], F_LAPMU*nsb)); };
-    auto dir_dot = [&](int k, real ax, real ay, real az) -> real {
-      real s = 0;
-      if (CX[k] > 0) s += ax; else if (CX[k] < 0) s -= ax;
-      if (CY[k] > 0) s += ay; else if (CY[k] < 0) s -= ay;
-      if (CZ[k] > 0) s += az; else if (CZ[k] < 0) s -= az;
-      return s;
-    };
-    auto pushk = [&](real v, rsrc_t r, int k) {
-      push<real, HBB, OPEN>(v, r, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
-    };
-    if constexpr (!MRT) {
-      const real drcx = drho*e_dcx*CS2, drcy = drho*e_dcy*CS2, drcz = drho*e_dcz*CS2;
-      const real e_fgx = mu*e_dcx + bfx, e_fgy = mu*e_dcy + bfy, e_fgz = mu*e_dcz + bfz;
-      const real a_over_rho = ALPHA/rho;
-      const real e_fhx = e_dcx - c*a_over_rho*(e_dpx - e_fgx), e_fhy = e_dcy - c*a_over_rho*(e_dpy - e_fgy), e_fhz = e_dcz - c*a_over_rho*(e_dpz - e_fgz);
-      const real u_sq = ux*ux + uy*uy + uz*uz;
-      const real u_fh_e = ux*e_fhx + uy*e_fhy + uz*e_fhz;
-      const real u_drhocd = ux*drcx + uy*drcy + uz*drcz;
-      const real u_fg_e = ux*e_fgx + uy*e_fgy + uz*e_fgz;
-      const real drho_cs2 = drho*CS2;
-      const real drmx = drho_cs2*f_dcx, drmy = drho_cs2*f_dcy, drmz = drho_cs2*f_dcz;
-      const real c_over = c/CS2/rho;
-      const real f_fgx = mu*f_dcx + bfx, f_fgy = mu*f_dcy + bfy, f_fgz = mu*f_dcz + bfz;
-      const real f_fhx = f_dcx - c_over*(f_dpx - f_fgx), f_fhy = f_dcy - c_over*(f_dpy - f_fgy), f_fhz = f_dcz - c_over*(f_dpz - f_fgz);
-      const real u_fh_f = ux*f_fhx + uy*f_fhy + uz*f_fhz;
-      const real u_drm = ux*drmx + uy*drmy + uz*drmz;
-      const real u_fg_f = ux*f_fgx + uy*f_fgy + uz*f_fgz;
+    const bool nostream = eb >= 0 && (eb & ENT_NOSTREAM);
+    if (nostream) {
+      const real oc = ldg(ovp, 0), orho = ldg(ovp, 1), op = ldg(ovp, 2);
+      const real oux = ldg(ovp, 3), ouy = ldg(ovp, 4), ouz = ldg(ovp, 5);
       #pragma unroll
       for (int k = 0; k < Q; ++k) {
-        const real w = W<real>(k);
-        real uk = dir_dot(k, ux, uy, uz);
-        real u_term = uk*(ALPHA + BETA_C*uk) - GAMMA_C*u_sq;
-        real G = w*((real)1 + u_term);
-        real f_proj = dir_dot(k, bfx, bfy, bfz);
-        const real cd_c = cdc[k], cd_p = cdp[k];
-        real drho_dk = drho*cd_c*CS2;
-        real eh = c*G;
-        eh -= (real)0.5*G*(cd_c - c*a_over_rho*(cd_p - mu*cd_c - f_proj));
-        eh += (real)0.5*G*u_fh_e;
-        real eg = w*(p + rho*CS2*u_term);
-        eg -= (real)0.5*(drho_dk*(G - w) + (mu*cd_c + f_proj)*G);
-        eg += (real)0.5*((G - w)*u_drhocd + G*u_fg_e);
-        real dGk = G - w;
-        real dc_dk = (real)0.5*(cd_c + bdc[k]), dp_dk = (real)0.5*(cd_p + bdp[k]);
-        real drho_dk2 = drho_cs2*dc_dk;
-        real ffk = mu*dc_dk + f_proj;
-        real fh = G*(dc_dk - c_over*(dp_dk - ffk) - u_fh_f + mob*avg_of(k));
-        real fg = (drho_dk2 - u_drm)*dGk + (ffk - u_fg_f)*G;
-        real gnew = gk[k] + relax*(eg - gk[k]) + fg;
-        pushk(eh + fh, rH, k);
-        pushk(gnew, rGo, k);
+        real hv, gv;
+        plain_eq<real>(k, oux, ouy, ouz, op, orho, oc, hv, gv);
+        push<real, HBB, OPEN>(hv, rH, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+        push<real, HBB, OPEN>(gv, rGo, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+      }
+      if (stats) {
+        double vx = (double)oux, vy = (double)ouy, vz = (double)ouz;
+        double v2 = vx*vx + vy*vy + vz*vz;
+        fx_add(s0, vx); fx_add(s1, vy); fx_add(s2, vz); fx_add(s3, v2); fx_add(s4, (double)oc);
+        vmax = (unsigned long long)__double_as_longlong(v2);
       }
     } else {
-      const real drho_cs2 = drho*CS2, rho_cs2 = rho*CS2, c_over = c/rho_cs2;
-      const real drcx = drho_cs2*e_dcx, drcy = drho_cs2*e_dcy, drcz = drho_cs2*e_dcz;
-      const real e_fgx = mu*e_dcx + bfx, e_fgy = mu*e_dcy + bfy, e_fgz = mu*e_dcz + bfz;
-      const real e_fhx = e_dcx - c_over*(e_dpx - e_fgx), e_fhy = e_dcy - c_over*(e_dpy - e_fgy), e_fhz = e_dcz - c_over*(e_dpz - e_fgz);
-      const real gamma_u_sq = GAMMA_C*(ux*ux + uy*uy + uz*uz);
-      const real u_term_h = ux*e_fhx + uy*e_fhy + uz*e_fhz;
-      const real u_force_g = ux*e_fgx + uy*e_fgy + uz*e_fgz;
-      const real u_drho = ux*drcx + uy*drcy + uz*drcz;
-      const real drmx = drho_cs2*f_dcx, drmy = drho_cs2*f_dcy, drmz = drho_cs2*f_dcz;
-      const real f_fgx = mu*f_dcx + bfx, f_fgy = mu*f_dcy + bfy, f_fgz = mu*f_dcz + bfz;
-      const real f_fhx = f_dcx - c_over*(f_dpx - f_fgx), f_fhy = f_dcy - c_over*(f_dpy - f_fgy), f_fhz = f_dcz - c_over*(f_dpz - f_fgz);
-      const real u_fh_f = ux*f_fhx + uy*f_fhy + uz*f_fhz;
-      const real u_drm = ux*drmx + uy*drmy + uz*drmz;
-      const real u_fg_f = ux*f_fgx + uy*f_fgy + uz*f_fgz;
-      // relaxation rates (device_engine.cuh:622-625, :640-643)
-      const real s0r = relax; const real taur = (real)1/s0r - (real)0.5;
-      const real s1r = (real)2*taur/((real)2*d.mrt_lambda + taur);
-      // x = S M (eq_g - g), accumulated column by column in the reference's
-      // order as each direction is produced, keeping its (s*M)*y rounding
-      real xx[Q];
-      #pragma unroll
-      for (int m = 0; m < Q; ++m) xx[m] = 0;
-      real fgs[Q];
-      #pragma unroll
-      for (int k = 0; k < Q; ++k) {
-        const real w = W<real>(k);
-        real uk = dir_dot(k, ux, uy, uz);
-        real u_term = uk*(ALPHA + BETA_C*uk) - gamma_u_sq;
-        real G = w*((real)1 + u_term);
-        real w_u_term = G - w;
-        real f_proj = dir_dot(k, bfx, bfy, bfz);
-        const real cd_c = cdc[k], cd_p = cdp[k];
-        real drho_dk = drho_cs2*cd_c;
-        real eh = c*G + (real)0.5*G*(u_term_h - cd_c + c_over*(cd_p - mu*cd_c - f_proj));
-        real eg = w*p + rho_cs2*w_u_term + (real)0.5*((u_drho - drho_dk)*w_u_term + (u_force_g + mu*cd_c + f_proj)*G);
-        real dGk = G - w;
-        real dc_dk = (real)0.5*(cd_c + bdc[k]), dp_dk = (real)0.5*(cd_p + bdp[k]);
-        real drho_dk2 = drho_cs2*dc_dk;
-        real ffk = mu*dc_dk + f_proj;
-        real fh = G*(dc_dk - c_over*(dp_dk - ffk) - u_fh_f + mob*avg_of(k));
-        fgs[k] = (drho_dk2 - u_drm)*dGk + (ffk - u_fg_f)*G;
-        pushk(eh + fh, rH, k);
-        const real yv = eg - gk[k];
+      real cN[Q], pN[Q], c2[Q], p2[Q];
+      {
+        unsigned vo2[Q];
+        ring2<real, OPEN>(d, rW, nb, io, vo2, fl2);
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
-          if (MRTTAB.M[m][k] != 0.0 && m != 3 && m != 5 && m != 7) {
-            const bool odd = (m == 4 || m == 6 || m == 8 || m >= 16);
-            const real sm = odd ? s1r : s0r;
-            xx[m] += sm*(real)MRTTAB.M[m][k]*yv;
-          }
+          cN[m] = bld<real>(rF, vo[m], F_C*nsb); pN[m] = bld<real>(rF, vo[m], F_PS*nsb);
+          c2[m] = bld<real>(rF, vo2[m], F_C*nsb); p2[m] = bld<real>(rF, vo2[m], F_PS*nsb);
         }
       }
+      const real c = bld<real>(rF, io, F_C*nsb), p = bld<real>(rF, io, F_PS*nsb);
+      const real mu = bld<real>(rF, io, F_MU*nsb), lmu = bld<real>(rF, io, F_LAPMU*nsb);
+      const real drho = d.drho;
+      const real CS2 = (real)(1.0/3.0), ALPHA = (real)3.0, GAMMA_C = (real)1.5, BETA_C = (real)4.5;
+      const real rho = ovp ? ldg(ovp, 1) : d.rho1 + drho*c;
+      const real crl = ovp ? ldg(ovp, 6) : c;     // relax from the raw c (k_moments)
+      const real cp = crl < (real)0 ? (real)0 : (crl > (real)1 ? (real)1 : crl);
+      const real tau = (real)1/(cp/d.tau0 + ((real)1 - cp)/d.tau1);
+      const real relax = (real)1/(tau + (real)0.5);
+      real gk[Q];
+      real jx = 0, jy = 0, jz = 0;
       #pragma unroll
-      for (int k = 0; k < Q; ++k) {
-        real acc = 0;
+      for (int k = 0; k < Q; ++k) { gk[k] = bld<real>(rGi, io, (unsigned)k*nsb); acc_vec<real>(k, gk[k], jx, jy, jz); }
+      // per-direction central and biased values, and the vector gradients in
+      // the reference's direction order
+      real cdc[Q], cdp[Q], bdc[Q], bdp[Q];
+      real e_dcx = 0, e_dcy = 0, e_dcz = 0, e_dpx = 0, e_dpy = 0, e_dpz = 0;
+      real b_cx = 0, b_cy = 0, b_cz = 0, b_px = 0, b_py = 0, b_pz = 0;
+      cdc[0] = cdp[0] = bdc[0] = bdp[0] = 0;
+      #pragma unroll
+      for (int m = 1; m < Q; ++m) {
+        const int o = OPP[m];
+        const bool hin = (fl >> m) & 1u, hip = (fl >> o) & 1u, h2 = (fl2 >> m) & 1u, hp2 = (fl2 >> o) & 1u;
+        bool use;
+        real nc = cd_num<real, HBB>(hin, hip, cN[m], cN[o], use);
+        real np = cd_num<real, HBB>(hin, hip, pN[m], pN[o], use);
+        const real coef = (real)3.0*(real)0.5*W<real>(m), base = (real)3.0*W<real>(m);
+        acc_vec<real>(m, use ? coef*nc : (real)0, e_dcx, e_dcy, e_dcz);
+        acc_vec<real>(m, use ? coef*np : (real)0, e_dpx, e_dpy, e_dpz);
+        cdc[m] = use ? (real)0.5*nc : (real)0;
+        cdp[m] = use ? (real)0.5*np : (real)0;
+        real vc = bd_num<real, HBB>(hin, h2, hip, hp2, cN[m], c2[m], cN[o], c2[o], c);
+        real vp = bd_num<real, HBB>(hin, h2, hip, hp2, pN[m], p2[m], pN[o], p2[o], p);
+        acc_vec<real>(m, base*vc, b_cx, b_cy, b_cz);
+        acc_vec<real>(m, base*vp, b_px, b_py, b_pz);
+        bdc[m] = vc; bdp[m] = vp;
+      }
+      const real f_dcx = (real)0.5*(e_dcx + b_cx), f_dcy = (real)0.5*(e_dcy + b_cy), f_dcz = (real)0.5*(e_dcz + b_cz);
+      const real f_dpx = (real)0.5*(e_dpx + b_px), f_dpy = (real)0.5*(e_dpy + b_py), f_dpz = (real)0.5*(e_dpz + b_pz);
+      // u* = u + 1/2 (g + f ff/rho + (mu/rho) grad_cd c)   (k_vel_press_corr)
+      const real inv = (real)1/(CS2*rho);
+      real ux = jx*inv, uy = jy*inv, uz = jz*inv;
+      if (eb >= 0 && (eb & ENT_BCPROC)) {
+        ux = ldg(ovp, 3); uy = ldg(ovp, 4); uz = ldg(ovp, 5);
+      } else {
+        real mor = mu/rho;
+        ux = ux + (real)0.5*(d.gx + d.fx*d.ff/rho + mor*e_dcx);
+        uy = uy + (real)0.5*(d.gy + d.fy*d.ff/rho + mor*e_dcy);
+        uz = uz + (real)0.5*(d.gz + d.fz*d.ff/rho + mor*e_dcz);
+      }
+      if (eb >= 0 && (eb & ENT_EQ)) {
+        // k_open_bnd overwrote g with the equilibrium of the node's fields
         #pragma unroll
-        for (int m = 0; m < Q; ++m) {
-          const bool cons = (m == 0 || m == 3 || m == 5 || m == 7);
-          if (MRTTAB.MI[k][m] != 0.0 && !cons) acc += (real)MRTTAB.MI[k][m]*xx[m];
+        for (int k = 0; k < Q; ++k) { real hv; plain_eq<real>(k, ux, uy, uz, p, rho, c, hv, gk[k]); }
+      }
+      if (stats) {                  // k_flow_stats, device_engine.cuh:660-683
+        double vx = (double)ux, vy = (double)uy, vz = (double)uz;
+        double v2 = vx*vx + vy*vy + vz*vz;
+        fx_add(s0, vx); fx_add(s1, vy); fx_add(s2, vz); fx_add(s3, v2); fx_add(s4, (double)c);
+        vmax = (unsigned long long)__double_as_longlong(v2);
+      }
+      const real bfx = rho*d.gx + d.fx*d.ff, bfy = rho*d.gy + d.fy*d.ff, bfz = rho*d.gz + d.fz*d.ff;
+      const real mob = d.mobility;
+      // avg_dir: 1/2 (lmu + lapmu(x+e_k)); a non-site neighbour fetched the own
+      // value, and 1/2 (lmu + lmu) == lmu exactly, the reference's self case
+      auto avg_of = [&](int k) -> real { return (real)0.5*(lmu + bld<real>(rF, vo[k], F_LAPMU*nsb)); };
+      auto dir_dot = [&](int k, real ax, real ay, real az) -> real {
+        real s = 0;
+        if (CX[k] > 0) s += ax; else if (CX[k] < 0) s -= ax;
+        if (CY[k] > 0) s += ay; else if (CY[k] < 0) s -= ay;
+        if (CZ[k] > 0) s += az; else if (CZ[k] < 0) s -= az;
+        return s;
+      };
+      auto pushk = [&](real v, rsrc_t r, int k) {
+        push<real, HBB, OPEN>(v, r, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+      };
+      if constexpr (!MRT) {
+        const real drcx = drho*e_dcx*CS2, drcy = drho*e_dcy*CS2, drcz = drho*e_dcz*CS2;
+        const real e_fgx = mu*e_dcx + bfx, e_fgy = mu*e_dcy + bfy, e_fgz = mu*e_dcz + bfz;
+        const real a_over_rho = ALPHA/rho;
+        const real e_fhx = e_dcx - c*a_over_rho*(e_dpx - e_fgx), e_fhy = e_dcy - c*a_over_rho*(e_dpy - e_fgy), e_fhz = e_dcz - c*a_over_rho*(e_dpz - e_fgz);
+        const real u_sq = ux*ux + uy*uy + uz*uz;
+        const real u_fh_e = ux*e_fhx + uy*e_fhy + uz*e_fhz;
+        const real u_drhocd = ux*drcx + uy*drcy + uz*drcz;
+        const real u_fg_e = ux*e_fgx + uy*e_fgy + uz*e_fgz;
+        const real drho_cs2 = drho*CS2;
+        const real drmx = drho_cs2*f_dcx, drmy = drho_cs2*f_dcy, drmz = drho_cs2*f_dcz;
+        const real c_over = c/CS2/rho;
+        const real f_fgx = mu*f_dcx + bfx, f_fgy = mu*f_dcy + bfy, f_fgz = mu*f_dcz + bfz;
+        const real f_fhx = f_dcx - c_over*(f_dpx - f_fgx), f_fhy = f_dcy - c_over*(f_dpy - f_fgy), f_fhz = f_dcz - c_over*(f_dpz - f_fgz);
+        const real u_fh_f = ux*f_fhx + uy*f_fhy + uz*f_fhz;
+        const real u_drm = ux*drmx + uy*drmy + uz*drmz;
+        const real u_fg_f = ux*f_fgx + uy*f_fgy + uz*f_fgz;
+        #pragma unroll
+        for (int k = 0; k < Q; ++k) {
+          const real w = W<real>(k);
+          real uk = dir_dot(k, ux, uy, uz);
+          real u_term = uk*(ALPHA + BETA_C*uk) - GAMMA_C*u_sq;
+          real G = w*((real)1 + u_term);
+          real f_proj = dir_dot(k, bfx, bfy, bfz);
+          const real cd_c = cdc[k], cd_p = cdp[k];
+          real drho_dk = drho*cd_c*CS2;
+          real eh = c*G;
+          eh -= (real)0.5*G*(cd_c - c*a_over_rho*(cd_p - mu*cd_c - f_proj));
+          eh += (real)0.5*G*u_fh_e;
+          real eg = w*(p + rho*CS2*u_term);
+          eg -= (real)0.5*(drho_dk*(G - w) + (mu*cd_c + f_proj)*G);
+          eg += (real)0.5*((G - w)*u_drhocd + G*u_fg_e);
+          real dGk = G - w;
+          real dc_dk = (real)0.5*(cd_c + bdc[k]), dp_dk = (real)0.5*(cd_p + bdp[k]);
+          real drho_dk2 = drho_cs2*dc_dk;
+          real ffk = mu*dc_dk + f_proj;
+          real fh = G*(dc_dk - c_over*(dp_dk - ffk) - u_fh_f + mob*avg_of(k));
+          real fg = (drho_dk2 - u_drm)*dGk + (ffk - u_fg_f)*G;
+          real gnew = gk[k] + relax*(eg - gk[k]) + fg;
+          pushk(eh + fh, rH, k);
+          pushk(gnew, rGo, k);
         }
-        pushk(gk[k] + acc + fgs[k], rGo, k);
+      } else {
+        const real drho_cs2 = drho*CS2, rho_cs2 = rho*CS2, c_over = c/rho_cs2;
+        const real drcx = drho_cs2*e_dcx, drcy = drho_cs2*e_dcy, drcz = drho_cs2*e_dcz;
+        const real e_fgx = mu*e_dcx + bfx, e_fgy = mu*e_dcy + bfy, e_fgz = mu*e_dcz + bfz;
+        const real e_fhx = e_dcx - c_over*(e_dpx - e_fgx), e_fhy = e_dcy - c_over*(e_dpy - e_fgy), e_fhz = e_dcz - c_over*(e_dpz - e_fgz);
+        const real gamma_u_sq = GAMMA_C*(ux*ux + uy*uy + uz*uz);
+        const real u_term_h = ux*e_fhx + uy*e_fhy + uz*e_fhz;
+        const real u_force_g = ux*e_fgx + uy*e_fgy + uz*e_fgz;
+        const real u_drho = ux*drcx + uy*drcy + uz*drcz;
+        const real drmx = drho_cs2*f_dcx, drmy = drho_cs2*f_dcy, drmz = drho_cs2*f_dcz;
+        const real f_fgx = mu*f_dcx + bfx, f_fgy = mu*f_dcy + bfy, f_fgz = mu*f_dcz + bfz;
+        const real f_fhx = f_dcx - c_over*(f_dpx - f_fgx), f_fhy = f_dcy - c_over*(f_dpy - f_fgy), f_fhz = f_dcz - c_over*(f_dpz - f_fgz);
+        const real u_fh_f = ux*f_fhx + uy*f_fhy + uz*f_fhz;
+        const real u_drm = ux*drmx + uy*drmy + uz*drmz;
+        const real u_fg_f = ux*f_fgx + uy*f_fgy + uz*f_fgz;
+        // relaxation rates (device_engine.cuh:622-625, :640-643)
+        const real s0r = relax; const real taur = (real)1/s0r - (real)0.5;
+        const real s1r = (real)2*taur/((real)2*d.mrt_lambda + taur);
+        // x = S M (eq_g - g), accumulated column by column in the reference's
+        // order as each direction is produced, keeping its (s*M)*y rounding
+        real xx[Q];
+        #pragma unroll
+        for (int m = 0; m < Q; ++m) xx[m] = 0;
+        real fgs[Q];
+        #pragma unroll
+        for (int k = 0; k < Q; ++k) {
+          const real w = W<real>(k);
+          real uk = dir_dot(k, ux, uy, uz);
+          real u_term = uk*(ALPHA + BETA_C*uk) - gamma_u_sq;
+          real G = w*((real)1 + u_term);
+          real w_u_term = G - w;
+          real f_proj = dir_dot(k, bfx, bfy, bfz);
+          const real cd_c = cdc[k], cd_p = cdp[k];
+          real drho_dk = drho_cs2*cd_c;
+          real eh = c*G + (real)0.5*G*(u_term_h - cd_c + c_over*(cd_p - mu*cd_c - f_proj));
+          real eg = w*p + rho_cs2*w_u_term + (real)0.5*((u_drho - drho_dk)*w_u_term + (u_force_g + mu*cd_c + f_proj)*G);
+          real dGk = G - w;
+          real dc_dk = (real)0.5*(cd_c + bdc[k]), dp_dk = (real)0.5*(cd_p + bdp[k]);
+          real drho_dk2 = drho_cs2*dc_dk;
+          real ffk = mu*dc_dk + f_proj;
+          real fh = G*(dc_dk - c_over*(dp_dk - ffk) - u_fh_f + mob*avg_of(k));
+          fgs[k] = (drho_dk2 - u_drm)*dGk + (ffk - u_fg_f)*G;
+          pushk(eh + fh, rH, k);
+          const real yv = eg - gk[k];
+          #pragma unroll
+          for (int m = 1; m < Q; ++m) {
+            if (MRTTAB.M[m][k] != 0.0 && m != 3 && m != 5 && m != 7) {
+              const bool odd = (m == 4 || m == 6 || m == 8 || m >= 16);
+              const real sm = odd ? s1r : s0r;
+              xx[m] += sm*(real)MRTTAB.M[m][k]*yv;
+            }
+          }
+        }
+        #pragma unroll
+        for (int k = 0; k < Q; ++k) {
+          real acc = 0;
+          #pragma unroll
+          for (int m = 0; m < Q; ++m) {
+            const bool cons = (m == 0 || m == 3 || m == 5 || m == 7);
+            if (MRTTAB.MI[k][m] != 0.0 && !cons) acc += (real)MRTTAB.MI[k][m]*xx[m];
+          }
+          pushk(gk[k] + acc + fgs[k], rGo, k);
+        }
       }
     }
   }

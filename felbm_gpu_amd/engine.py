@@ -16,6 +16,12 @@ from ._lib import check
 
 D3Q19_W = np.array([1.0 / 3.0] + [1.0 / 18.0] * 6 + [1.0 / 36.0] * 12)
 
+# open-boundary settings (felbm_gpu_main.cu:340-364); keys of felbm_open_bnd
+OPEN_BND_DEFAULTS = {"use_in_vel": 0, "use_in_prs": 0, "use_in_fluid": 0, "use_out_prs": 0, "use_out_fluid": 0,
+                     "u_in": (0.0, 0.0, 0.0), "p_in": 1.0, "p_out": 1.0, "c_out_fixed": 0.0, "rho_out_fixed": 1.0,
+                     "inlet_c_a": 1.0, "inlet_mode": 0, "split_axis": 0, "inlet_period": 0.0, "inlet_duty": 0.5,
+                     "inlet_ramp": 0.0, "split_pos": 0.0}
+
 
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
@@ -161,6 +167,30 @@ class MultiPhaseGPU:
 
     def set_forcing_factor(self, ff):
         check(self.lib.felbm_set_forcing_factor(self.h, self.slot, float(ff)))
+
+    # :369-400, with the DevParams open-BC block of felbm_gpu_main.cu:340-364
+    def set_open_bnd(self, ob, inlet, outlet, icoord=None, buf_verts=(), buf_refs=(), copy_to_buffers=False):
+        f = dict(OPEN_BND_DEFAULTS)
+        f.update(ob)
+        s = _lib.OpenBnd()
+        for key in ("use_in_vel", "use_in_prs", "use_in_fluid", "use_out_prs", "use_out_fluid", "inlet_mode",
+                    "split_axis"):
+            setattr(s, key, int(f[key]))
+        for a in range(3):
+            s.u_in[a] = float(f["u_in"][a])
+        for key in ("p_in", "p_out", "c_out_fixed", "rho_out_fixed", "inlet_c_a", "inlet_period", "inlet_duty",
+                    "inlet_ramp", "split_pos"):
+            setattr(s, key, float(f[key]))
+        inl = np.ascontiguousarray(inlet, dtype=np.int32)
+        out = np.ascontiguousarray(outlet, dtype=np.int32)
+        bv = np.ascontiguousarray(buf_verts, dtype=np.int32)
+        br = np.ascontiguousarray(buf_refs, dtype=np.int32)
+        ic = None if icoord is None else np.ascontiguousarray(icoord, dtype=np.float64)
+        pi = C.POINTER(C.c_int32)
+        check(self.lib.felbm_set_open_bnd(self.h, self.slot, C.byref(s), inl.ctypes.data_as(pi), len(inl),
+                                          out.ctypes.data_as(pi), len(out), None if ic is None else _dptr(ic),
+                                          bv.ctypes.data_as(pi), br.ctypes.data_as(pi), len(bv),
+                                          int(bool(copy_to_buffers))))
 
     def run(self, t_begin, t_end, timedep="constant", period=1.0):
         """The driver's stepping loop (src/felbm_gpu_main.cu:701-749 without

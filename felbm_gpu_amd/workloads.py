@@ -82,6 +82,37 @@ def slab_c(solid, axis=0, lo=0.0, hi=None, width=4.0):
     return 0.5 * (1.0 + np.tanh(2.0 * d / width))
 
 
+def open_channel(solid, empty_layers=2, copy_to_buffers=True, split_axis=0):
+    """Open inlet/outlet lists for flow along y (the reference's SubDomain
+    builds them in felbm_local, which is not available; this is the build's own
+    rule).  The first and last `empty_layers` y-planes are cleared of solid.
+    Without buffers the inlet is the plane y=0 and the outlet y=Ny-1; with
+    copy_to_buffers the planes y=0 and y=Ny-1 are buffer nodes mirroring the
+    same (x, z) node of y=1 and y=Ny-2, which are the inlet and outlet.
+    Returns (solid, inlet, outlet, icoord, buf_verts, buf_refs); icoord is each
+    inlet node's coordinate along split_axis."""
+    s = np.array(solid, dtype=np.uint8, copy=True)
+    nz, ny, nx = s.shape
+    e = int(empty_layers)
+    s[:, :e, :] = 0
+    s[:, ny - e:, :] = 0
+    x, y, z = site_coords(s)
+    idx = -np.ones((nz, ny, nx), dtype=np.int64)
+    idx[z, y, x] = np.arange(len(x))
+    coords = (x, y, z)
+    yin, yout = (1, ny - 2) if copy_to_buffers else (0, ny - 1)
+    inlet = np.nonzero(y == yin)[0].astype(np.int32)
+    outlet = np.nonzero(y == yout)[0].astype(np.int32)
+    icoord = coords[split_axis][inlet].astype(np.float64)
+    bv, br = np.zeros(0, np.int32), np.zeros(0, np.int32)
+    if copy_to_buffers:
+        b_in = np.nonzero(y == 0)[0]
+        b_out = np.nonzero(y == ny - 1)[0]
+        bv = np.concatenate([b_in, b_out]).astype(np.int32)
+        br = np.concatenate([idx[z[b_in], 1, x[b_in]], idx[z[b_out], ny - 2, x[b_out]]]).astype(np.int32)
+    return s, inlet, outlet, icoord, bv, br
+
+
 def workload(name):
     """Geometry, params, switches and IC for a named workload."""
     if name == "2d_cylinders_128":

@@ -85,6 +85,18 @@ def load():
     return lib
 
 
+OPEN_BND_DEFAULTS = {"use_in_vel": 0, "use_in_prs": 0, "use_in_fluid": 0, "use_out_prs": 0, "use_out_fluid": 0,
+                     "u_in": (0.0, 0.0, 0.0), "p_in": 1.0, "p_out": 1.0, "c_out_fixed": 0.0, "rho_out_fixed": 1.0,
+                     "inlet_c_a": 1.0, "inlet_mode": 0, "split_axis": 0, "inlet_period": 0.0, "inlet_duty": 0.5,
+                     "inlet_ramp": 0.0, "split_pos": 0.0}
+
+
+def open_bnd_fields(ob):
+    out = dict(OPEN_BND_DEFAULTS)
+    out.update(ob)
+    return out
+
+
 def forcing_factor_at(timedep, period, t):
     return load().orc_forcing_factor_at(timedep.encode(), float(period), int(t))
 
@@ -107,12 +119,22 @@ class Oracle:
     """FP64 CPU restatement of MultiPhaseGPU (the checker)."""
 
     def __init__(self, solid, params, use_mrt=False, fuse_collision=True, stream_inplace=False, mass_mode=0,
-                 periodic=(1, 1, 1), use_halfway_bb=True):
+                 periodic=(1, 1, 1), use_halfway_bb=True, open_bnd=None):
         self.lib = load()
         self.solid = np.ascontiguousarray(solid, dtype=np.uint8)
         nz, ny, nx = self.solid.shape
         sw = OrcSwitches(int(use_mrt), int(fuse_collision), int(stream_inplace), int(mass_mode))
         self._p = to_orc_params(params)
+        if open_bnd is not None:
+            ob = open_bnd_fields(open_bnd)
+            self._p.open_bnd = 1
+            for key in ("use_in_vel", "use_in_prs", "use_in_fluid", "use_out_prs", "use_out_fluid", "inlet_mode",
+                        "split_axis"):
+                setattr(self._p, key, int(ob[key]))
+            self._p.u_in_x, self._p.u_in_y, self._p.u_in_z = ob["u_in"]
+            for key in ("p_in", "p_out", "c_out_fixed", "rho_out_fixed", "inlet_c_a", "inlet_period", "inlet_duty",
+                        "inlet_ramp", "split_pos"):
+                setattr(self._p, key, float(ob[key]))
         self.E = self.lib.orc_new(nx, ny, nz, self.solid.ctypes.data_as(C.POINTER(C.c_uint8)), int(use_halfway_bb),
                                   int(periodic[0]), int(periodic[1]), int(periodic[2]), C.byref(self._p), C.byref(sw))
         self.n = self.lib.orc_n(self.E)
@@ -184,6 +206,19 @@ class Oracle:
         arrs = [np.empty(s * n, dtype=np.int32) for s in sizes]
         self.lib.orc_get_tables(self.E, *[a.ctypes.data_as(_PI) for a in arrs])
         return dict(zip(names, arrs))
+
+    def set_open_bnd(self, inlet, outlet, icoord=None, buf_verts=(), buf_refs=(), copy_to_buffers=False):
+        inl = np.ascontiguousarray(inlet, dtype=np.int32)
+        out = np.ascontiguousarray(outlet, dtype=np.int32)
+        bv = np.ascontiguousarray(buf_verts, dtype=np.int32)
+        br = np.ascontiguousarray(buf_refs, dtype=np.int32)
+        ic = None if icoord is None else np.ascontiguousarray(icoord, dtype=np.float64)
+        self._keep = (inl, out, bv, br, ic)
+        pi = C.POINTER(C.c_int32)
+        rc = self.lib.orc_set_open_bnd(self.E, inl.ctypes.data_as(pi), len(inl), out.ctypes.data_as(pi), len(out),
+                                       None if ic is None else _d(ic), bv.ctypes.data_as(pi),
+                                       br.ctypes.data_as(pi), len(bv), int(copy_to_buffers))
+        assert rc == 0
 
     def set_forcing_factor(self, ff):
         self.lib.orc_set_forcing_factor(self.E, float(ff))

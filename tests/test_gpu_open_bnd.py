@@ -1,0 +1,143 @@
+"""GPU parity of the open inlet/outlet boundaries (multiphase_gpu.cuh:369-446,
+device_engine.cuh:699-846) against the FP64 oracle.
+
+Flow along y through a periodic-in-x,z sphere pack with two cleared layers at
+each end.  Direct mode overwrites the inlet/outlet distributions with the
+equilibrium of the imposed fields; copy_to_buffers mode imposes the fields on
+the inlet/outlet nodes and writes the equilibrium into the buffer nodes
+beyond them, which do not collide.  FP64 results are bit-identical except
+where the injection schedule evaluates tanh (ramp > 0), whose device and host
+libm results may differ in the last bit: there the bar is 1e-13.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from felbm_gpu_amd import MultiPhaseGPU, Batch, FelbmError
+from felbm_gpu_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PER = (1, 0, 1)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def channel(copy, seed=3, shape=(10, 24, 12)):
+    nz, ny, nx = shape
+    solid = W.spheres(nx, ny, nz, 2.0, 3.0, 0.7, seed, keep_largest=False, periodic=PER)
+    return W.open_channel(solid, 2, copy_to_buffers=copy, split_axis=0)
+
+
+def make_open_pair(copy, ob, use_mrt, precision="f64", c0=0.0, seed=3):
+    from oracle.pyoracle import Oracle
+    s, inl, out, ic, bv, br = channel(copy, seed)
+    prm = W.derive_params(theta_deg=60.0, acceleration=(0.0, 1e-6, 0.0))
+    o = Oracle(s, prm, use_mrt=use_mrt, periodic=PER, open_bnd=ob)
+    o.set_open_bnd(inl, out, ic, bv, br, copy)
+    g = MultiPhaseGPU(precision).init(s, prm, {"use_mrt": use_mrt, "correct_op_mass": 0}, periodic=PER)
+    g.set_open_bnd(ob, inl, out, ic, bv, br, copy)
+    x, y, z = W.site_coords(s)
+    cinit = np.where(y < s.shape[1] // 3, 1.0 - c0, c0).astype(np.float64)
+    o.init_rest(cinit, 1.0)
+    h, gg = o.download_state()
+    g.upload_state(h, gg)
+    return o, g
+
+
+def compare(o, g, tol=0.0):
+    ho, go = o.download_state()
+    hg, gg = g.download_state()
+    dh, dg = float(np.abs(ho - hg).max()), float(np.abs(go - gg).max())
+    fo, fg = o.download(), g.download()
+    df = {k: float(np.abs(fo[k] - fg[k]).max()) for k in fo}
+    _say("  max|dh| %.3e max|dg| %.3e fields %s" % (dh, dg, df))
+    assert dh <= tol and dg <= tol
+    for k, v in df.items():
+        assert v <= tol, k
+
+
+DIRECT = {"use_in_prs": 1, "use_in_fluid": 1, "use_out_prs": 1, "p_in": 1.0005, "p_out": 1.0, "inlet_c_a": 1.0}
+BUFFERED = {"use_in_vel": 1, "use_in_fluid": 1, "use_out_prs": 1, "use_out_fluid": 1, "u_in": (0.0, 2e-3, 0.0),
+            "p_out": 1.0, "c_out_fixed": 0.0, "rho_out_fixed": 1.0, "inlet_c_a": 1.0, "inlet_mode": 1,
+            "inlet_period": 12.0, "inlet_duty": 0.5, "inlet_ramp": 0.0}
+
+
+@pytest.mark.parametrize("use_mrt", [0, 1])
+def test_direct_open_boundaries_bitexact(use_mrt):
+    _say("direct open BC, mrt", use_mrt)
+    o, g = make_open_pair(False, DIRECT, use_mrt)
+    o.run(40)
+    g.step(40)
+    compare(o, g)
+
+
+@pytest.mark.parametrize("use_mrt", [0, 1])
+def test_buffered_open_boundaries_alternate_injection_bitexact(use_mrt):
+    _say("copy_to_buffers open BC, alternate injection, mrt", use_mrt)
+    o, g = make_open_pair(True, BUFFERED, use_mrt)
+    o.run(40)
+    g.step(40)
+    compare(o, g)
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_split_and_stripes_injection_with_ramp(mode):
+    ob = dict(BUFFERED, inlet_mode=mode, inlet_ramp=2.0, split_pos=5.0, inlet_period=6.0, inlet_duty=0.4)
+    _say("split/stripes injection mode", mode)
+    o, g = make_open_pair(True, ob, 1)
+    o.run(30)
+    g.step(30)
+    compare(o, g, tol=1e-13)
+
+
+def test_open_boundary_stats_and_fields():
+    o, g = make_open_pair(False, DIRECT, 1)
+    o.run(10)
+    g.step(10, stats_last=True)
+    so, sg = o.flow_stats(), g.flow_stats()
+    _say("stats", so, sg)
+    assert np.allclose(so, sg, rtol=1e-12, atol=1e-15)
+
+
+def test_open_boundaries_fp32_within_tolerance():
+    o, g = make_open_pair(True, BUFFERED, 1, precision="f32")
+    o.run(20)
+    g.step(20)
+    fo, fg = o.download(), g.download()
+    dc = float(np.abs(fo["c"] - fg["c"]).max())
+    du = max(float(np.abs(fo[k] - fg[k]).max()) for k in ("ux", "uy", "uz"))
+    _say("fp32 open BC: max|dc| %.3e max|du| %.3e" % (dc, du))
+    assert dc <= 1e-5 and du <= 1e-5
+
+
+def test_open_instance_in_batch_equals_alone():
+    s, inl, out, ic, bv, br = channel(True)
+    prm = W.derive_params(theta_deg=60.0)
+    sw = {"use_mrt": 1, "correct_op_mass": 0}
+    x, y, z = W.site_coords(s)
+    c0 = np.where(y < 8, 1.0, 0.0)
+    alone = MultiPhaseGPU("f64").init(s, prm, sw, periodic=PER)
+    alone.set_open_bnd(BUFFERED, inl, out, ic, bv, br, True)
+    alone.init_rest(c0, 1.0)
+    b = Batch(precision="f64")
+    other = MultiPhaseGPU(batch=b).init(W.spheres(16, 16, 16, 2.0, 3.0, 0.6, 9), prm, sw)
+    other.init_rest(np.full(other.n, 0.3), 1.0)
+    inb = MultiPhaseGPU(batch=b).init(s, prm, sw, periodic=PER)
+    inb.set_open_bnd(BUFFERED, inl, out, ic, bv, br, True)
+    inb.init_rest(c0, 1.0)
+    alone.step(25)
+    b.step(25)
+    assert np.array_equal(alone.download_state()[0], inb.download_state()[0])
+    assert np.array_equal(alone.download_state()[1], inb.download_state()[1])
+
+
+def test_open_boundaries_reject_mass_correction():
+    s, inl, out, ic, bv, br = channel(False)
+    g = MultiPhaseGPU("f64").init(s, W.derive_params(), {"use_mrt": 1, "correct_op_mass": 1}, periodic=PER)
+    with pytest.raises(FelbmError):
+        g.set_open_bnd(DIRECT, inl, out, ic)
