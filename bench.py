@@ -63,6 +63,9 @@ def main():
     ap.add_argument("--workload", default="porous_300")
     ap.add_argument("--cpu-steps", type=int, default=int(os.environ.get("FELBM_CPU_STEPS", "24")))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep-steps", type=int, default=1000,
+                    help="steps of the 48-instance batched sweep (0 skips it)")
+    ap.add_argument("--serve", type=int, default=16, help="requests of the served stream (0 skips it)")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="only run the device-to-device copy probe of this size (counter calibration)")
     args = ap.parse_args()
@@ -118,6 +121,24 @@ def main():
     mass_drift = abs(eng.current_mass() - eng.get_target_mass())
     print("[bench] %.1f MLUPS, kernels %s" % (mlups, json.dumps(kern)), file=sys.stderr, flush=True)
 
+    def log(msg):
+        print("[bench] " + msg, file=sys.stderr, flush=True)
+
+    sweep = None
+    if args.sweep_steps > 0:
+        eng.free()
+        from felbm_gpu_amd import serving
+        sweep = serving.run_sweep(W.sweep_instances(48, 1000), args.sweep_steps, "f32", log=log)
+        log("sweep: %s" % json.dumps(sweep))
+    served = None
+    if args.serve > 0:
+        eng.free()
+        from felbm_gpu_amd import serving
+        res = serving.run_serve(args.serve, rate=4.0, seed=2024, precision="f32", log=log)
+        served = {k: v for k, v in res.items() if k != "done"}
+        served["requests_steps"] = [d["steps"] for d in res["done"]]
+        log("serve: %s" % json.dumps(served))
+
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline (%d steps)" % args.cpu_steps, file=sys.stderr, flush=True)
@@ -137,6 +158,8 @@ def main():
         "mass_drift": mass_drift,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched_sweep": sweep,
+        "served": served,
     }
     print(json.dumps(out))
 

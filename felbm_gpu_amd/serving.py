@@ -1,0 +1,249 @@
+"""Many lattices on one GPU: the batched parameter sweep and the served
+request stream.
+
+sweep     N independent instances (edge, porosity, contact angle, mobility
+          drawn per instance) advanced together -- one launch per pass serves
+          all of them, the step loop captured in a HIP graph -- against the same
+          instances run one after another through a single-instance engine.
+          Batched MLUPS = sum_i n_i steps / wall seconds.
+
+serve     a stream of solve requests with Poisson arrivals.  Each request is an
+          open-boundary channel (inlet/outlet along y, two cleared layers and
+          buffer nodes at each end, copy_to_buffers, no mass correction), run
+          until the mean order parameter changes by less than `tol` (relative)
+          over one check interval or `max_steps` are done.  The server admits
+          arrived requests while the instances fit the HBM budget, advances all
+          active instances together `check_every` steps at a time, and retires
+          converged ones.  Latency = completion - arrival; aggregate MLUPS =
+          sum_i n_i steps_i / makespan.
+
+The scheduler (Server) talks to an engine through four calls -- admit, step,
+mean_c, retire -- so its policy is tested on the CPU with a stand-in engine.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import workloads as W
+
+
+def instance_bytes(n_sites, precision):
+    """Device bytes of one instance: h, g0, g1 (19 each), 13 fields, coords,
+    and the row words and accumulators (rounded up)."""
+    R = 8 if precision in ("f64", 8) else 4
+    return int(n_sites * (70 * R + 4) + (1 << 20))
+
+
+# ---- parameter sweep ----------------------------------------------------------------
+
+def sweep_problem(spec):
+    edge = int(spec["edge"])
+    solid = W.spheres(edge, edge, edge, 5.0, 9.0, spec["porosity"], spec["seed"])
+    prm = W.derive_params(sigma=0.01, width=4.0, mobility_coeff=spec["mobility_coeff"], theta_deg=spec["theta"],
+                          acceleration=(1e-6, 0.0, 0.0))
+    return solid, prm, W.slab_c(solid, 0, 0, edge // 2, 4.0)
+
+
+def run_sweep(specs, steps, precision="f32", graph_chunk=50, log=None):
+    """Batched vs sequential MLUPS over the same instances (MRT, mass
+    correction on, FP32 distributions with FP64 sums)."""
+    from .engine import Batch, MultiPhaseGPU
+    sw = {"use_mrt": 1, "correct_op_mass": 1}
+    t_gen = time.time()
+    problems = [sweep_problem(s) for s in specs]
+    if log:
+        log("sweep: %d instances generated in %.1fs" % (len(problems), time.time() - t_gen))
+    steps = max(graph_chunk, (int(steps) // graph_chunk) * graph_chunk)
+
+    # batched: one handle, every instance in the same launches, graph replay
+    b = Batch(precision=precision)
+    insts = []
+    for solid, prm, c0 in problems:
+        g = MultiPhaseGPU(batch=b).init(solid, prm, sw)
+        g.init_rest(c0, 1.0)
+        g.record_target_mass()
+        insts.append(g)
+    total = sum(g.n for g in insts)
+    b.step(2)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step_graph(graph_chunk, steps // graph_chunk)
+    b.synchronize()
+    t_batched = time.perf_counter() - t0
+    for g in insts:
+        g.free()
+    b.free()
+    if log:
+        log("sweep: batched %d sites x %d steps in %.3fs" % (total, steps, t_batched))
+
+    # sequential: the same instances one at a time through a one-instance engine
+    t_seq = 0.0
+    for solid, prm, c0 in problems:
+        g = MultiPhaseGPU(precision).init(solid, prm, sw)
+        g.init_rest(c0, 1.0)
+        g.record_target_mass()
+        g.step(2)
+        g.batch.synchronize()
+        t0 = time.perf_counter()
+        g.step(steps)
+        g.batch.synchronize()
+        t_seq += time.perf_counter() - t0
+        g.free()
+        g.batch.free()
+    if log:
+        log("sweep: sequential %.3fs" % t_seq)
+    lu = float(total) * steps
+    return {"instances": len(problems), "sites": int(total), "steps": steps,
+            "batched_MLUPS": lu / t_batched / 1e6, "sequential_MLUPS": lu / t_seq / 1e6,
+            "speedup": t_seq / t_batched, "batched_s": t_batched, "sequential_s": t_seq}
+
+
+# ---- served request stream ----------------------------------------------------------
+
+def make_requests(count, rate=4.0, seed=2024, edge_range=(48, 150), dp_range=(1e-4, 1e-3),
+                  theta_range=(30.0, 150.0), porosity=0.45):
+    """Seeded request stream: arrival offsets (s) of a Poisson process and the
+    per-request edge, pressure drop and contact angle."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        out.append({"id": i, "arrival": t, "edge": int(rng.integers(edge_range[0], edge_range[1] + 1)),
+                    "dp": float(rng.uniform(*dp_range)), "theta": float(rng.uniform(*theta_range)),
+                    "porosity": porosity, "seed": int(seed + 1 + i)})
+    return out
+
+
+def request_problem(req):
+    """Open channel for one request: sphere pack, flow along y, inlet phase
+    injected (c = 1), outlet drains at p = 1, uniform initial c = 0."""
+    e = int(req["edge"])
+    rmin, rmax = 3.0, 6.0
+    solid = W.spheres(e, e, e, rmin, rmax, req["porosity"], req["seed"], keep_largest=True, periodic=(1, 0, 1))
+    s, inl, out, ic, bv, br = W.open_channel(solid, 2, copy_to_buffers=True)
+    prm = W.derive_params(sigma=0.01, width=4.0, mobility_coeff=0.02, theta_deg=req["theta"])
+    ob = {"use_in_prs": 1, "use_in_fluid": 1, "use_out_prs": 1, "p_in": 1.0 + req["dp"], "p_out": 1.0,
+          "inlet_c_a": 1.0}
+    return {"solid": s, "params": prm, "open_bnd": ob, "inlet": inl, "outlet": out, "icoord": ic,
+            "buf_verts": bv, "buf_refs": br}
+
+
+class GpuBackend:
+    """The engine side of the server: one Batch, one slot per request."""
+
+    def __init__(self, precision="f32"):
+        from .engine import Batch
+        self.precision = precision
+        self.batch = Batch(precision=precision)
+
+    def admit(self, prob):
+        from .engine import MultiPhaseGPU
+        g = MultiPhaseGPU(batch=self.batch).init(prob["solid"], prob["params"], {"use_mrt": 1, "correct_op_mass": 0},
+                                                 periodic=(1, 0, 1))
+        g.set_open_bnd(prob["open_bnd"], prob["inlet"], prob["outlet"], prob["icoord"], prob["buf_verts"],
+                       prob["buf_refs"], True)
+        g.init_rest(np.zeros(g.n), 1.0)
+        return g, g.n
+
+    def step(self, nsteps):
+        self.batch.step(nsteps, stats_last=True)
+
+    def mean_c(self, g):
+        return float(g.flow_stats()[4]) / g.n
+
+    def retire(self, g):
+        g.free()
+
+    def close(self):
+        self.batch.free()
+
+
+class Server:
+    """Continuous batching with HBM admission control.
+
+    Each tick: admit every arrived request that fits the budget (FIFO), advance
+    all active instances by `check_every` steps, then retire those whose mean c
+    moved by less than `tol` relative to the previous check, or that reached
+    `max_steps`.  `clock` returns seconds since the stream started."""
+
+    def __init__(self, backend, hbm_budget=200e9, check_every=500, tol=1e-6, max_steps=20000, precision="f32",
+                 clock=None, sleep=None):
+        self.be = backend
+        self.budget = float(hbm_budget)
+        self.check_every = int(check_every)
+        self.tol = float(tol)
+        self.max_steps = int(max_steps)
+        self.precision = precision
+        self.clock = clock
+        self.sleep = sleep or time.sleep
+
+    def run(self, requests, problems, log=None):
+        t0 = time.perf_counter()
+        clock = self.clock or (lambda: time.perf_counter() - t0)
+        pending = sorted(requests, key=lambda r: r["arrival"])
+        active, done = [], []
+        used = 0.0
+        lu = 0.0
+        while pending or active:
+            now = clock()
+            # admission (FIFO, budget permitting)
+            while pending and pending[0]["arrival"] <= now:
+                r = pending[0]
+                need = instance_bytes(int(np.count_nonzero(problems[r["id"]]["solid"] == 0)), self.precision)
+                if used + need > self.budget and active:
+                    break
+                pending.pop(0)
+                h, n = self.be.admit(problems[r["id"]])
+                active.append({"req": r, "h": h, "n": n, "steps": 0, "last": None, "bytes": need,
+                               "admitted": clock()})
+                used += need
+            if not active:
+                if pending:
+                    self.sleep(max(0.0, pending[0]["arrival"] - clock()))
+                continue
+            self.be.step(self.check_every)
+            lu += sum(a["n"] for a in active) * float(self.check_every)
+            still = []
+            for a in active:
+                a["steps"] += self.check_every
+                m = self.be.mean_c(a["h"])
+                conv = a["last"] is not None and abs(m - a["last"]) <= self.tol * max(abs(m), 1e-300)
+                a["last"] = m
+                if conv or a["steps"] >= self.max_steps:
+                    t = clock()
+                    self.be.retire(a["h"])
+                    used -= a["bytes"]
+                    done.append({"id": a["req"]["id"], "arrival": a["req"]["arrival"], "done": t,
+                                 "latency": t - a["req"]["arrival"], "steps": a["steps"], "sites": a["n"],
+                                 "converged": bool(conv), "mean_c": m})
+                    if log:
+                        log("serve: request %d done after %d steps (%s), latency %.2fs"
+                            % (a["req"]["id"], a["steps"], "converged" if conv else "step cap", t - a["req"]["arrival"]))
+                else:
+                    still.append(a)
+            active = still
+        makespan = max(d["done"] for d in done) - min(d["arrival"] for d in done) if done else 0.0
+        lat = np.array([d["latency"] for d in done])
+        return {"requests": len(done), "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
+                "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
+                "aggregate_MLUPS": lu / makespan / 1e6 if makespan > 0 else None,
+                "makespan_s": makespan, "converged": int(sum(d["converged"] for d in done)),
+                "lattice_updates": lu, "done": done}
+
+
+def run_serve(count, rate=4.0, seed=2024, precision="f32", check_every=500, tol=1e-6, max_steps=20000,
+              hbm_budget=200e9, log=None):
+    reqs = make_requests(count, rate, seed)
+    t_gen = time.time()
+    problems = {r["id"]: request_problem(r) for r in reqs}
+    if log:
+        log("serve: %d requests generated in %.1fs" % (len(reqs), time.time() - t_gen))
+    be = GpuBackend(precision)
+    try:
+        res = Server(be, hbm_budget, check_every, tol, max_steps, precision).run(reqs, problems, log)
+    finally:
+        be.close()
+    res["rate_per_s"] = rate
+    return res

@@ -1,0 +1,110 @@
+"""Scheduler policy of the served request stream, on the CPU with a stand-in
+engine and a simulated clock: every request completes, convergence and the
+step cap retire instances, admission is FIFO and never exceeds the HBM
+budget while another instance is active, and the latency and throughput
+figures follow from the clock."""
+import numpy as np
+
+from felbm_gpu_amd import serving as S
+
+
+class FakeClock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += max(0.0, dt)
+
+
+class FakeBackend:
+    """mean c of request i approaches 1 as 1 - r_i^k after k checks; one step
+    of one site costs 1e-9 s of simulated time."""
+
+    def __init__(self, clock, rates, sizes):
+        self.clock, self.rates, self.sizes = clock, rates, sizes
+        self.active = {}
+        self.log = []
+
+    def admit(self, prob):
+        rid = prob["id"]
+        self.active[rid] = 0
+        self.log.append(("admit", rid, self.clock()))
+        return rid, self.sizes[rid]
+
+    def step(self, nsteps):
+        self.clock.t += 1e-9 * nsteps * sum(self.sizes[r] for r in self.active)
+        for r in self.active:
+            self.active[r] += 1
+
+    def mean_c(self, rid):
+        return 1.0 - self.rates[rid] ** self.active[rid]
+
+    def retire(self, rid):
+        del self.active[rid]
+        self.log.append(("retire", rid, self.clock()))
+
+
+def make(count, sizes):
+    reqs = S.make_requests(count, rate=4.0, seed=5)
+    probs = {r["id"]: {"id": r["id"], "solid": np.zeros(sizes[r["id"]], np.uint8)} for r in reqs}
+    return reqs, probs
+
+
+def test_requests_arrive_as_a_seeded_poisson_stream():
+    a = S.make_requests(2000, rate=4.0, seed=1)
+    b = S.make_requests(2000, rate=4.0, seed=1)
+    assert a == b
+    gaps = np.diff([0.0] + [r["arrival"] for r in a])
+    assert abs(gaps.mean() - 0.25) < 0.02 and np.all(gaps > 0)
+    edges = np.array([r["edge"] for r in a])
+    assert edges.min() >= 48 and edges.max() <= 150
+    dps = np.array([r["dp"] for r in a])
+    assert dps.min() >= 1e-4 and dps.max() <= 1e-3
+
+
+def test_every_request_completes_and_retires_on_convergence_or_cap():
+    n = 12
+    sizes = {i: 100000 + 5000 * i for i in range(n)}
+    rates = {i: (0.001 if i % 3 else 0.999999) for i in range(n)}   # every third never converges
+    clk = FakeClock()
+    be = FakeBackend(clk, rates, sizes)
+    reqs, probs = make(n, sizes)
+    res = S.Server(be, hbm_budget=1e12, check_every=500, tol=1e-6, max_steps=5000, clock=clk,
+                   sleep=clk.sleep).run(reqs, probs)
+    assert res["requests"] == n
+    by_id = {d["id"]: d for d in res["done"]}
+    for i in range(n):
+        d = by_id[i]
+        if i % 3:
+            assert d["converged"] and d["steps"] < 5000
+        else:
+            assert not d["converged"] and d["steps"] == 5000
+        assert d["latency"] >= 0.0 and d["done"] >= d["arrival"]
+    assert res["p50_latency_s"] <= res["p99_latency_s"]
+    assert res["lattice_updates"] == sum(d["sites"] * d["steps"] for d in res["done"])
+    assert abs(res["aggregate_MLUPS"] - res["lattice_updates"] / res["makespan_s"] / 1e6) < 1e-9
+
+
+def test_admission_is_fifo_and_respects_the_hbm_budget():
+    n = 10
+    sizes = {i: 400000 for i in range(n)}
+    per = S.instance_bytes(400000, "f32")
+    budget = 3.5 * per                     # at most three at a time
+    clk = FakeClock()
+    be = FakeBackend(clk, {i: 0.5 for i in range(n)}, sizes)
+    reqs, probs = make(n, sizes)
+    S.Server(be, hbm_budget=budget, check_every=500, tol=1e-6, max_steps=20000, clock=clk,
+             sleep=clk.sleep).run(reqs, probs)
+    live, peak, order = set(), 0, []
+    for ev, rid, t in be.log:
+        if ev == "admit":
+            live.add(rid)
+            order.append(rid)
+        else:
+            live.discard(rid)
+        peak = max(peak, len(live))
+    assert peak == 3
+    assert order == sorted(order)
