@@ -119,6 +119,24 @@ def main():
             "alg_bytes_per_launch": ALG_VALUES["collide"] * bytes_per_val * n,
             "copy_probe_GBs": copy_gbs}
     mass_drift = abs(eng.current_mass() - eng.get_target_mass())
+
+    # the same lattice carrying 12,000 tracers on the two slab interfaces, six
+    # diffusivities x 2000, each plane carrying every diffusivity
+    tr_mlups = None
+    if args.workload == "porous_300":
+        edge = w["solid"].shape[2]
+        pos, _ = W.seed_planes(w["solid"], 6000, (0.0, edge / 2.0), axis=0, seed=12)
+        ladder = "1e-4:1000, 3.98e-4:1000, 1.58e-3:1000, 6.31e-3:1000, 2.51e-2:1000, 1e-1:1000"
+        dm = np.concatenate([W.dm_groups(ladder), W.dm_groups(ladder)])
+        eng.tracers_init(pos, dm, seed=3, substeps=1, velocity_skip=1)
+        eng.step(5)
+        eng.batch.synchronize()
+        t1 = time.perf_counter()
+        eng.step(args.steps)
+        eng.batch.synchronize()
+        tr_mlups = n * args.steps / (time.perf_counter() - t1) / 1e6
+        eng.tracers_init(np.zeros((0, 3)), np.zeros(0))
+        print("[bench] with 12,000 tracers: %.1f MLUPS" % tr_mlups, file=sys.stderr, flush=True)
     print("[bench] %.1f MLUPS, kernels %s" % (mlups, json.dumps(kern)), file=sys.stderr, flush=True)
 
     def log(msg):
@@ -156,6 +174,7 @@ def main():
                    "hbm_frac_total": mlups * 1e6 * B_ALG_PER_LU / (HBM_PEAK_GBS * 1e9)},
         "kernels_ms": {k: v["ms"] for k, v in kern.items()},
         "mass_drift": mass_drift,
+        "with_tracers_MLUPS": tr_mlups,
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched_sweep": sweep,

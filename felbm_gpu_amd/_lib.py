@@ -68,6 +68,9 @@ SIGNATURES = {
     "felbm_step_graph": (_I, [_P, _I, _I]),
     "felbm_set_forcing_factor": (_I, [_P, _I, _D]),
     "felbm_forcing_factor_at": (_I, [C.c_char_p, _D, C.c_int64, C.POINTER(C.c_double)]),
+    "felbm_tracers_init": (_I, [_P, _I, C.c_int64, _PD, _PD, C.c_uint64, _I, _I]),
+    "felbm_tracers_download": (_I, [_P, _I, _PD, _PD, _PI32]),
+    "felbm_tracers_count": (_I, [_P, _I, C.POINTER(C.c_int64)]),
     "felbm_synchronize": (_I, [_P]),
     "felbm_record_target_mass": (_I, [_P, _I]),
     "felbm_get_target_mass": (_I, [_P, _I, _PD]),

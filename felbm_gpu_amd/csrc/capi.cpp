@@ -57,6 +57,14 @@ int felbm_step(felbm_engine *e, int nsteps, int stats_last) { ENG(e); return e->
 int felbm_step_graph(felbm_engine *e, int nsteps, int replays) { ENG(e); return e->impl->step_graph(nsteps, replays); }
 int felbm_set_forcing_factor(felbm_engine *e, int slot, double ff) { ENG(e); return e->impl->set_forcing_factor(slot, ff); }
 
+int felbm_tracers_init(felbm_engine *e, int slot, int64_t np, const double *xyz, const double *dm, uint64_t seed,
+                       int substeps, int velocity_skip)
+{ ENG(e); return e->impl->tracers_init(slot, np, xyz, dm, seed, substeps, velocity_skip); }
+int felbm_tracers_download(felbm_engine *e, int slot, double *xyz, double *uvw, int32_t *blocked)
+{ ENG(e); return e->impl->tracers_download(slot, xyz, uvw, blocked); }
+int felbm_tracers_count(felbm_engine *e, int slot, int64_t *np)
+{ ENG(e); if (!np) { fm::set_error("null output pointer"); return FELBM_EINVAL; } return e->impl->tracers_count(slot, np); }
+
 // forcing_factor_at of the reference driver (src/felbm_gpu_main.cu:587-594):
 // the time-dependence name is matched as a substring, anything else is constant
 int felbm_forcing_factor_at(const char *timedep, double period, int64_t t, double *ff)

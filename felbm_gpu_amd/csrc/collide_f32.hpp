@@ -39,6 +39,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
   if (threadIdx.x == 0 && bm.y == 0) {
     if (d.mass_on) stg(d.acc, ACC_FLAGS, 1ull);
     if (OPEN && d.obc) stg(d.acc, ACC_FLAGS + 1, ldg(d.acc, ACC_FLAGS + 1) + 1ull);   // ++open_step
+    if (d.trk) stg(d.acc, ACC_FLAGS + 2, ldg(d.acc, ACC_FLAGS + 2) + 1ull);            // ++tracer step
   }
   if (live) {
     constexpr unsigned RB = 4;

@@ -15,6 +15,7 @@
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "collide_f32.hpp"
+#include "tracers.hpp"
 
 namespace fm {
 
@@ -48,6 +49,12 @@ struct Slot {
   char *ob_base = nullptr;        // ent [n] + ov [entries*8]
   std::vector<ObEntry> entries;   // desc filled at rebuild
   int e0 = 0;                     // first entry in the engine-wide list
+  // tracers
+  int tr_np = 0, tr_substeps = 1;
+  unsigned long long tr_seed = 0;
+  char *tr_base = nullptr;        // pos [3np] f64, dm [np] f64, uvw [3np] f64, blocked [np] i32, tu [3ns] real
+  double *tr_pos = nullptr, *tr_dm = nullptr, *tr_uvw = nullptr;
+  int *tr_blocked = nullptr;
 };
 
 template <typename real>
@@ -163,6 +170,7 @@ public:
     FM_HIP(hipStreamSynchronize(stream_));
     FM_HIP(hipFree(slots_[slot].base));
     if (slots_[slot].ob_base) FM_HIP(hipFree(slots_[slot].ob_base));
+    if (slots_[slot].tr_base) FM_HIP(hipFree(slots_[slot].tr_base));
     slots_[slot] = Slot<real>();
     dirty_ = true;
     return rebuild();
@@ -456,6 +464,62 @@ public:
     return FELBM_OK;
   }
 
+  // ---- tracers (ParticleManager, felbm_gpu_main.cu:426-438, :752-777) ------------
+  int tracers_init(int slot, int64_t np, const double *xyz, const double *dm, uint64_t seed, int substeps,
+                   int vskip) override {
+    if (int e = check(slot)) return e;
+    if (np < 0 || np > (1 << 30) || (np && (!xyz || !dm)) || substeps < 1 || vskip < 1) {
+      set_error("invalid tracer arguments"); return FELBM_EINVAL;
+    }
+    Slot<real> &S = slots_[slot];
+    FM_HIP(hipStreamSynchronize(stream_));
+    if (S.tr_base) { FM_HIP(hipFree(S.tr_base)); S.tr_base = nullptr; }
+    S.tr_np = 0; S.d.trk = 0; S.d.tu = nullptr;
+    dirty_desc_ = true;
+    if (np == 0) return FELBM_OK;
+    const size_t n = (size_t)np, ns = (size_t)S.ns;
+    size_t off = 0;
+    auto take = [&](size_t b) { size_t o = off; off = align_up(off + b, 256); return o; };
+    const size_t o_p = take(3 * n * 8), o_d = take(n * 8), o_u = take(3 * n * 8), o_b = take(n * 4);
+    const size_t o_t = take(3 * ns * sizeof(real));
+    if (hipMalloc((void **)&S.tr_base, off) != hipSuccess) { set_error("device allocation failed"); return FELBM_ENOMEM; }
+    FM_HIP(hipMemsetAsync(S.tr_base, 0, off, stream_));
+    S.tr_pos = (double *)(S.tr_base + o_p); S.tr_dm = (double *)(S.tr_base + o_d);
+    S.tr_uvw = (double *)(S.tr_base + o_u); S.tr_blocked = (int *)(S.tr_base + o_b);
+    FM_HIP(hipMemcpyAsync(S.tr_pos, xyz, 3 * n * 8, hipMemcpyHostToDevice, stream_));
+    FM_HIP(hipMemcpyAsync(S.tr_dm, dm, n * 8, hipMemcpyHostToDevice, stream_));
+    static const unsigned long long zero = 0;
+    FM_HIP(hipMemcpyAsync(S.d.acc + ACC_FLAGS + 2, &zero, sizeof(zero), hipMemcpyHostToDevice, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    S.tr_np = (int)np; S.tr_seed = seed; S.tr_substeps = substeps;
+    S.d.trk = 1; S.d.tr_vskip = vskip; S.d.tu = (real *)(S.tr_base + o_t);
+    return FELBM_OK;
+  }
+
+  int tracers_download(int slot, double *xyz, double *uvw, int32_t *blocked) override {
+    if (int e = check(slot)) return e;
+    Slot<real> &S = slots_[slot];
+    if (!S.tr_np) { set_error("no tracers on this slot"); return FELBM_ESTATE; }
+    if (int e = flush()) return e;
+    const size_t n = (size_t)S.tr_np;
+    if (uvw) {
+      hipLaunchKernelGGL(k_tracer_velocity<real>, dim3((S.tr_np + 255) / 256), dim3(256), 0, stream_, d_desc_,
+                         slot_desc_[slot], (const double *)S.tr_pos, S.tr_uvw, S.tr_np);
+      FM_LAUNCH();
+      FM_HIP(hipMemcpyAsync(uvw, S.tr_uvw, 3 * n * 8, hipMemcpyDeviceToHost, stream_));
+    }
+    if (xyz) FM_HIP(hipMemcpyAsync(xyz, S.tr_pos, 3 * n * 8, hipMemcpyDeviceToHost, stream_));
+    if (blocked) FM_HIP(hipMemcpyAsync(blocked, S.tr_blocked, n * 4, hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    return FELBM_OK;
+  }
+
+  int tracers_count(int slot, int64_t *np) override {
+    if (int e = check(slot)) return e;
+    *np = slots_[slot].tr_np;
+    return FELBM_OK;
+  }
+
   int kernel_times(int nsteps, float *ms4) override {
     if (int e = flush()) return e;
     if (nmap_ == 0 || nsteps <= 0) { for (int q = 0; q < 4; ++q) ms4[q] = 0; return FELBM_OK; }
@@ -518,6 +582,7 @@ private:
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     for (auto &s : slots_) if (s.used && s.base) (void)hipFree(s.base);
     for (auto &s : slots_) if (s.used && s.ob_base) (void)hipFree(s.ob_base);
+    for (auto &s : slots_) if (s.used && s.tr_base) (void)hipFree(s.tr_base);
     if (d_ent_all_) (void)hipFree(d_ent_all_);
     slots_.clear();
     if (d_desc_) (void)hipFree(d_desc_);
@@ -659,6 +724,15 @@ private:
     for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_key(k, 2, dim3(nvar_[k]), d_var_[k], stats); FM_LAUNCH(); }
     if (timed) FM_HIP(hipEventRecord(ev_[2], stream_));
     for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_key(k, 3, dim3(nvar_[k]), d_var_[k], stats); FM_LAUNCH(); }
+    // tracers move on this step's u snapshot before the collision pass
+    // advances the step counter they key their draws with
+    for (size_t q = 0; q < slots_.size(); ++q) {
+      Slot<real> &S = slots_[q];
+      if (!S.used || !S.tr_np) continue;
+      hipLaunchKernelGGL(k_tracers<real>, dim3((S.tr_np + 255) / 256), blk, 0, stream_, d_desc_, slot_desc_[q],
+                         S.tr_pos, (const double *)S.tr_dm, S.tr_blocked, S.tr_np, S.tr_seed, S.tr_substeps);
+      FM_LAUNCH();
+    }
     if (timed) FM_HIP(hipEventRecord(ev_[3], stream_));
     for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_key(k, 4, dim3(nvar_[k]), d_var_[k], stats); FM_LAUNCH(); }
     if (timed) FM_HIP(hipEventRecord(ev_[4], stream_));

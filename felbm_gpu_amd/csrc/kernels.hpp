@@ -97,6 +97,10 @@ struct Inst {
   int use_in_vel, use_in_prs, use_in_fluid, use_out_prs, use_out_fluid, inlet_mode;
   double u_in[3], p_in, p_out, c_out_fixed, rho_out_fixed;
   double inlet_c_a, inlet_period, inlet_duty, inlet_ramp, split_pos;
+  // tracers: the velocity snapshot they are advected on
+  int trk;                     // tracers present
+  int tr_vskip;                // refresh the snapshot every tr_vskip steps
+  real *tu;                    // [3][ns] snapshot of u (the field pass's u*)
 };
 
 // site classes of the open boundary
@@ -112,7 +116,7 @@ struct ObEntry { int desc, site, ref, rkind, eid, pad; double rcoord; };
 // accumulator layout (unsigned long long, NSLOT slots per quantity)
 constexpr int ACC_MASS = 0;                 // [parity][5][NSLOT]: M.hi M.lo W.hi W.lo bad
 constexpr int ACC_STATS = 10*NSLOT;         // [12][NSLOT]: 5 sums x (hi,lo), max bits, bad
-constexpr int ACC_FLAGS = 22*NSLOT;         // [0] pending correction, [1] open-boundary step
+constexpr int ACC_FLAGS = 22*NSLOT;         // [0] pending correction, [1] open-boundary step, [2] tracer step
 constexpr int ACC_MISC = 22*NSLOT + 8;      // [5][NSLOT] single-instance reductions
 constexpr int ACC_WORDS = 27*NSLOT + 8;
 
@@ -614,6 +618,13 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
     const long ns = d.ns;
     stg(exp, i, rho); stg(exp, ns + i, ux); stg(exp, 2*ns + i, uy); stg(exp, 3*ns + i, uz);
   }
+  if (!exp && d.trk) {          // tracers read this step's u (felbm_gpu_main.cu:752-763)
+    const unsigned long long ts = ldg(d.acc, ACC_FLAGS + 2);
+    if (ts % (unsigned long long)d.tr_vskip == 0) {
+      const long ns = d.ns;
+      stg(d.tu, i, ux); stg(d.tu, ns + i, uy); stg(d.tu, 2*ns + i, uz);
+    }
+  }
 }
 
 // ==== K3: lap(mu) (k_pack_lap_zero + k_lap_mf) ======================================
@@ -682,6 +693,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
   if (threadIdx.x == 0 && bm.y == 0) {
     if (d.mass_on) stg(d.acc, ACC_FLAGS, 1ull);                   // correction owed from now on
     if (OPEN && d.obc) stg(d.acc, ACC_FLAGS + 1, ldg(d.acc, ACC_FLAGS + 1) + 1ull);   // ++open_step
+    if (d.trk) stg(d.acc, ACC_FLAGS + 2, ldg(d.acc, ACC_FLAGS + 2) + 1ull);            // ++tracer step
   }
   if (live) {
     constexpr unsigned RB = sizeof(real);

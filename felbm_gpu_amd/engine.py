@@ -192,6 +192,24 @@ class MultiPhaseGPU:
                                           bv.ctypes.data_as(pi), br.ctypes.data_as(pi), len(bv),
                                           int(bool(copy_to_buffers))))
 
+    # tracers: ParticleManager (felbm_gpu_main.cu:426-438, :752-777)
+    def tracers_init(self, xyz, dm, seed=0, substeps=1, velocity_skip=1):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        dm = np.ascontiguousarray(dm, dtype=np.float64).reshape(-1)
+        assert len(dm) == len(xyz)
+        check(self.lib.felbm_tracers_init(self.h, self.slot, len(dm), _dptr(xyz), _dptr(dm), int(seed),
+                                          int(substeps), int(velocity_skip)))
+
+    def tracers_download(self):
+        np_ = C.c_int64()
+        check(self.lib.felbm_tracers_count(self.h, self.slot, C.byref(np_)))
+        n = np_.value
+        xyz, uvw = np.empty((n, 3)), np.empty((n, 3))
+        blk = np.empty(n, dtype=np.int32)
+        check(self.lib.felbm_tracers_download(self.h, self.slot, _dptr(xyz), _dptr(uvw),
+                                              blk.ctypes.data_as(C.POINTER(C.c_int32))))
+        return xyz, uvw, blk
+
     def run(self, t_begin, t_end, timedep="constant", period=1.0):
         """The driver's stepping loop (src/felbm_gpu_main.cu:701-749 without
         I/O): for t in [t_begin, t_end) set forcing_factor_at(t), step, apply

@@ -113,6 +113,48 @@ def open_channel(solid, empty_layers=2, copy_to_buffers=True, split_axis=0):
     return s, inlet, outlet, icoord, bv, br
 
 
+def dm_groups(spec):
+    """Per-tracer diffusivities from particles_dm_groups = "Dm:count, ..."
+    (README.md:428-433)."""
+    out = []
+    for part in str(spec).split(","):
+        part = part.strip()
+        if not part:
+            continue
+        dm, cnt = part.split(":")
+        out += [float(dm)] * int(cnt)
+    return np.array(out, dtype=np.float64)
+
+
+def seed_planes(solid, per_plane, offsets, axis=0, seed=0):
+    """Tracers seeded uniformly on planes normal to `axis` at the given offsets
+    (particles_init_mode = plane_*, README.md:435-442): each plane resamples
+    until its quota of points whose nearest voxel is fluid is reached.
+    Returns positions (n, 3) as (x, y, z) and the plane id of each."""
+    nz, ny, nx = solid.shape
+    dims = (nx, ny, nz)
+    rng = np.random.default_rng(seed)
+    pos, pid = [], []
+    for k, off in enumerate(offsets):
+        got = []
+        need = int(per_plane)
+        while need > 0:
+            cand = np.empty((max(2 * need, 64), 3))
+            for a in range(3):
+                cand[:, a] = rng.uniform(-0.5, dims[a] - 0.5, len(cand))
+            cand[:, axis] = float(off)
+            v = np.rint(cand).astype(np.int64)
+            for a in range(3):
+                v[:, a] = np.clip(v[:, a], 0, dims[a] - 1)
+            ok = solid[v[:, 2], v[:, 1], v[:, 0]] == 0
+            take = cand[ok][:need]
+            got.append(take)
+            need -= len(take)
+        pos.append(np.concatenate(got))
+        pid += [k] * int(per_plane)
+    return np.concatenate(pos), np.array(pid, dtype=np.int32)
+
+
 def workload(name):
     """Geometry, params, switches and IC for a named workload."""
     if name == "2d_cylinders_128":

@@ -144,6 +144,20 @@ int  felbm_set_open_bnd(felbm_engine *e, int slot, const felbm_open_bnd *ob,
                         const int32_t *buf_verts, const int32_t *buf_refs, int n_buf,
                         int copy_to_buffers);
 
+/* ---- tracers: ParticleManager construction, update() and output_state()
+ * (felbm_gpu_main.cu:426-438, :752-777; README.md:421-464).  np tracers at
+ * positions xyz[3*np] (voxel (x,y,z) at position (x,y,z)) with molecular
+ * diffusivities dm[np]; each step advances them by `substeps` Euler substeps
+ * on the step's velocity, refreshed every velocity_skip steps
+ * (particles_velocity_skip), with a Brownian kick sqrt(2 Dm dt) per axis from
+ * the counter-based stream `seed`.  np = 0 removes them. */
+int  felbm_tracers_init(felbm_engine *e, int slot, int64_t np, const double *xyz, const double *dm,
+                        uint64_t seed, int substeps, int velocity_skip);
+/* positions [3*np], interpolated velocities [3*np] and per-tracer counts of
+ * rejected (wall-blocked) substeps [np]; any pointer may be NULL */
+int  felbm_tracers_download(felbm_engine *e, int slot, double *xyz, double *uvw, int32_t *blocked);
+int  felbm_tracers_count(felbm_engine *e, int slot, int64_t *np);
+
 /* ---- verification helpers -------------------------------------------------
  * Reference-format tables (multiphase_gpu.cuh:93-208) regenerated on the GPU
  * from the engine's own neighbour logic, for bitwise comparison. Any pointer
