@@ -25,8 +25,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic values per site per launch (FP32 -> x4 bytes); see DESIGN.md
-ALG_VALUES = {"moments": 43, "cfield": 8, "mufield": 2, "collide": 61}
-B_ALG_PER_LU = 452  # bytes per lattice update, FP32 (113 values)
+# (the mu pass also packs c, p*, lapmu, mu into one record per site: +2 read, +4 written)
+ALG_VALUES = {"moments": 43, "cfield": 8, "mufield": 7, "collide": 61}
+B_ALG_PER_LU = 452  # compulsory bytes per lattice update, FP32 (113 values)
 
 
 def _profile_traffic(workload):

@@ -76,18 +76,19 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
         vmax = (unsigned long long)__double_as_longlong(v2);
       }
     } else {
-      float cN[Q], pN[Q], c2[Q], p2[Q];
+      float cN[Q], pN[Q], c2[Q], p2[Q], lN[Q];
       {
         unsigned vo2[Q];
         ring2<float, OPEN>(d, rW, nb, io, vo2, fl2);
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
-          cN[m] = bld<float>(rF, vo[m], F_C*nsb); pN[m] = bld<float>(rF, vo[m], F_PS*nsb);
-          c2[m] = bld<float>(rF, vo2[m], F_C*nsb); p2[m] = bld<float>(rF, vo2[m], F_PS*nsb);
+          const Rec4<float> r1 = bld_rec4((const float *)nullptr, rF, vo[m], F_REC*nsb);
+          const Rec2<float> r2 = bld_rec2((const float *)nullptr, rF, vo2[m], F_REC*nsb);
+          cN[m] = r1.c; pN[m] = r1.p; lN[m] = r1.l; c2[m] = r2.c; p2[m] = r2.p;
         }
       }
-      const float c = bld<float>(rF, io, F_C*nsb), p = bld<float>(rF, io, F_PS*nsb);
-      const float mu = bld<float>(rF, io, F_MU*nsb), lmu = bld<float>(rF, io, F_LAPMU*nsb);
+      const Rec4<float> r0 = bld_rec4((const float *)nullptr, rF, io, F_REC*nsb);
+      const float c = r0.c, p = r0.p, mu = r0.m, lmu = r0.l;
       const float drho = d.drho;
       const float CS2 = 1.0f/3.0f;
       const float rho = ovp ? ldg(ovp, 1) : d.rho1 + drho*c;
@@ -181,7 +182,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
         const float wut = w*u_term;
         const float G = w + wut;
         // 1/2 (lmu + lapmu(x+e_k)); a non-site neighbour fetched lmu itself
-        const float avgk = 0.5f*(lmu + bld<float>(rF, vo[k], F_LAPMU*nsb));
+        const float avgk = 0.5f*(lmu + (k ? lN[k] : lmu));
         const float A = K0 + a2*bdc[k] + a4*bdp[k] + a5*fp + mob*avgk;
         pushk(G*A, rH, k);
         const float cdk = cdc[k], dc2 = cdk + bdc[k];

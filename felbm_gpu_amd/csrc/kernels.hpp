@@ -208,13 +208,73 @@ template <> __device__ __forceinline__ double bld<double>(rsrc_t r, unsigned vo,
 { return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0)); }
 template <> __device__ __forceinline__ uint64_t bld<uint64_t>(rsrc_t r, unsigned vo, unsigned so)
 { return __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0)); }
+// streaming loads read once per step: non-temporal, so they do not displace
+// the stencil fields the next passes re-read from L2
+template <typename T> __device__ __forceinline__ T bld_nt(rsrc_t r, unsigned vo, unsigned so);
+template <> __device__ __forceinline__ float bld_nt<float>(rsrc_t r, unsigned vo, unsigned so)
+{ return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 2)); }
+template <> __device__ __forceinline__ double bld_nt<double>(rsrc_t r, unsigned vo, unsigned so)
+{ return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 2)); }
 __device__ __forceinline__ void bst(float v, rsrc_t r, unsigned vo, unsigned so)
 { __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vo, (int)so, 0); }
 __device__ __forceinline__ void bst(double v, rsrc_t r, unsigned vo, unsigned so)
 { __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, (int)vo, (int)so, 0); }
 
-// field block layout: [craw, p0, jx, jy, jz, c, mu, ps, lapmu] x ns
-enum { F_CRAW = 0, F_P0 = 1, F_JX = 2, F_JY = 3, F_JZ = 4, F_C = 5, F_MU = 6, F_PS = 7, F_LAPMU = 8, F_COUNT = 13 };
+// field block layout: [craw, p0, jx, jy, jz, c, mu, ps, lapmu] x ns, then
+// 4 x ns holding either the per-site records [c, p*, lapmu, mu] the mu-stencil
+// pass writes for the collision (one vector load per neighbour), or, between
+// steps, the exported rho, ux, uy, uz of a field download
+enum { F_CRAW = 0, F_P0 = 1, F_JX = 2, F_JY = 3, F_JZ = 4, F_C = 5, F_MU = 6, F_PS = 7, F_LAPMU = 8, F_REC = 9,
+       F_COUNT = 13 };
+
+// record loads/stores: (c, p*) or (c, p*, lapmu, mu) of one site, vo = site
+// byte offset in a scalar field (i * sizeof(real)); the record sits at 4*vo
+// The loaded vector is bit-cast as a whole: a bit-cast of one element of the
+// builtin's result (bit_cast<float>(v[1])) compiles to a read of element 0 with
+// this toolchain, which narrows the load to a single dword.
+template <typename real> struct Rec2 { real c, p; };
+template <typename real> struct Rec4 { real c, p, l, m; };
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ Rec2<float> bld_rec2(const float *, rsrc_t r, unsigned vo, unsigned so)
+{
+  const f32x2 f = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(vo*4u), (int)so, 0));
+  return { f[0], f[1] };
+}
+__device__ __forceinline__ Rec4<float> bld_rec4(const float *, rsrc_t r, unsigned vo, unsigned so)
+{
+  const f32x4 f = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(vo*4u), (int)so, 0));
+  return { f[0], f[1], f[2], f[3] };
+}
+__device__ __forceinline__ Rec2<double> bld_rec2(const double *, rsrc_t r, unsigned vo, unsigned so)
+{
+  auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(vo*4u), (int)so, 0);
+  typedef __attribute__((ext_vector_type(2))) unsigned u2;
+  return { __builtin_bit_cast(double, u2{v[0], v[1]}), __builtin_bit_cast(double, u2{v[2], v[3]}) };
+}
+__device__ __forceinline__ Rec4<double> bld_rec4(const double *, rsrc_t r, unsigned vo, unsigned so)
+{
+  auto a = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(vo*4u), (int)so, 0);
+  auto b = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(vo*4u + 16u), (int)so, 0);
+  typedef __attribute__((ext_vector_type(2))) unsigned u2;
+  return { __builtin_bit_cast(double, u2{a[0], a[1]}), __builtin_bit_cast(double, u2{a[2], a[3]}),
+           __builtin_bit_cast(double, u2{b[0], b[1]}), __builtin_bit_cast(double, u2{b[2], b[3]}) };
+}
+__device__ __forceinline__ void bst_rec4(float c, float p, float l, float m, rsrc_t r, unsigned vo, unsigned so)
+{
+  typedef __attribute__((ext_vector_type(4))) unsigned u4;
+  u4 v = { __builtin_bit_cast(unsigned, c), __builtin_bit_cast(unsigned, p),
+           __builtin_bit_cast(unsigned, l), __builtin_bit_cast(unsigned, m) };
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(vo*4u), (int)so, 0);
+}
+__device__ __forceinline__ void bst_rec4(double c, double p, double l, double m, rsrc_t r, unsigned vo, unsigned so)
+{
+  typedef __attribute__((ext_vector_type(2))) unsigned u2;
+  typedef __attribute__((ext_vector_type(4))) unsigned u4;
+  u2 cc = __builtin_bit_cast(u2, c), pp = __builtin_bit_cast(u2, p), ll = __builtin_bit_cast(u2, l), mm = __builtin_bit_cast(u2, m);
+  __builtin_amdgcn_raw_buffer_store_b128(u4{cc[0], cc[1], pp[0], pp[1]}, r, (int)(vo*4u), (int)so, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(u4{ll[0], ll[1], mm[0], mm[1]}, r, (int)(vo*4u + 16u), (int)so, 0);
+}
 
 // ---- lattice lookup ---------------------------------------------------------------
 // The wrapped coordinates x-2..x+2 (and y, z) of one site, computed once;
@@ -387,7 +447,7 @@ __global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ 
     real c = 0, p = 0, ux = 0, uy = 0, uz = 0;
     #pragma unroll
     for (int k = 0; k < Q; ++k) {
-      real hv = bld<real>(rH, io, (unsigned)k*nsb), gv = bld<real>(rG, io, (unsigned)k*nsb);
+      real hv = bld_nt<real>(rH, io, (unsigned)k*nsb), gv = bld_nt<real>(rG, io, (unsigned)k*nsb);
       c += hv; p += gv;
       acc_vec<real>(k, gv, ux, uy, uz);
     }
@@ -652,7 +712,7 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
   #pragma unroll
   for (int m = 1; m < Q; ++m)
     s += lap_term<real, HBB>((fl >> m) & 1u, (fl >> OPP[m]) & 1u, mn[m], mn[OPP[m]], m0, (real)0, (real)3.0*W<real>(m));
-  bst(s, rF, io, F_LAPMU*nsb);
+  bst_rec4(bld<real>(rF, io, F_C*nsb), bld<real>(rF, io, F_PS*nsb), s, m0, rF, io, F_REC*nsb);
 }
 
 // Push one post-collision population k of the site at byte offset io: into
@@ -730,18 +790,19 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
         vmax = (unsigned long long)__double_as_longlong(v2);
       }
     } else {
-      real cN[Q], pN[Q], c2[Q], p2[Q];
+      real cN[Q], pN[Q], c2[Q], p2[Q], lN[Q];
       {
         unsigned vo2[Q];
         ring2<real, OPEN>(d, rW, nb, io, vo2, fl2);
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
-          cN[m] = bld<real>(rF, vo[m], F_C*nsb); pN[m] = bld<real>(rF, vo[m], F_PS*nsb);
-          c2[m] = bld<real>(rF, vo2[m], F_C*nsb); p2[m] = bld<real>(rF, vo2[m], F_PS*nsb);
+          const Rec4<real> r1 = bld_rec4((const real *)nullptr, rF, vo[m], F_REC*nsb);
+          const Rec2<real> r2 = bld_rec2((const real *)nullptr, rF, vo2[m], F_REC*nsb);
+          cN[m] = r1.c; pN[m] = r1.p; lN[m] = r1.l; c2[m] = r2.c; p2[m] = r2.p;
         }
       }
-      const real c = bld<real>(rF, io, F_C*nsb), p = bld<real>(rF, io, F_PS*nsb);
-      const real mu = bld<real>(rF, io, F_MU*nsb), lmu = bld<real>(rF, io, F_LAPMU*nsb);
+      const Rec4<real> r0 = bld_rec4((const real *)nullptr, rF, io, F_REC*nsb);
+      const real c = r0.c, p = r0.p, mu = r0.m, lmu = r0.l;
       const real drho = d.drho;
       const real CS2 = (real)(1.0/3.0), ALPHA = (real)3.0, GAMMA_C = (real)1.5, BETA_C = (real)4.5;
       const real rho = ovp ? ldg(ovp, 1) : d.rho1 + drho*c;
@@ -805,7 +866,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
       const real mob = d.mobility;
       // avg_dir: 1/2 (lmu + lapmu(x+e_k)); a non-site neighbour fetched the own
       // value, and 1/2 (lmu + lmu) == lmu exactly, the reference's self case
-      auto avg_of = [&](int k) -> real { return (real)0.5*(lmu + bld<real>(rF, vo[k], F_LAPMU*nsb)); };
+      auto avg_of = [&](int k) -> real { return (real)0.5*(lmu + (k ? lN[k] : lmu)); };
       auto dir_dot = [&](int k, real ax, real ay, real az) -> real {
         real s = 0;
         if (CX[k] > 0) s += ax; else if (CX[k] < 0) s -= ax;

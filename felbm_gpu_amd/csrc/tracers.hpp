@@ -67,11 +67,13 @@ template <typename real>
 __device__ inline void interp_u(const Inst<real> &d, const double (&p)[3], double (&u)[3])
 {
   #pragma clang fp contract(off)
+  u[0] = 0.0; u[1] = 0.0; u[2] = 0.0;
+  // a position off the lattice (non-finite, or beyond int range) reads no voxel
+  for (int a = 0; a < 3; ++a) if (!(fabs(p[a]) < 1e9)) return;
   const double fl0 = floor(p[0]), fl1 = floor(p[1]), fl2 = floor(p[2]);
   const int x0 = (int)fl0, y0 = (int)fl1, z0 = (int)fl2;
   const double fx = p[0] - fl0, fy = p[1] - fl1, fz = p[2] - fl2;
   const long ns = d.ns;
-  u[0] = 0.0; u[1] = 0.0; u[2] = 0.0;
   for (int cz = 0; cz < 2; ++cz)
     for (int cy = 0; cy < 2; ++cy)
       for (int cx = 0; cx < 2; ++cx) {
@@ -114,16 +116,19 @@ __global__ void __launch_bounds__(256) k_tracers(const Inst<real> *__restrict__ 
       if (d.Nz > 1) q[2] += amp*xi[2];
     }
     bool ok = true;
-    int v[3];
+    int v[3] = { 0, 0, 0 };
     for (int a = 0; a < 3; ++a) {
+      if (!(fabs(q[a]) < 1e9)) { ok = false; continue; }   // non-finite or off the lattice
       if (P[a]) {
         const double L = (double)N[a];
         q[a] = q[a] - L*floor(q[a]/L);
       } else if (!(q[a] >= -0.5 && q[a] < (double)N[a] - 0.5)) {
         ok = false;
+        continue;
       }
       v[a] = (int)rint(q[a]);
       if (P[a] && v[a] >= N[a]) v[a] -= N[a];
+      if (v[a] < 0 || v[a] >= N[a]) ok = false;        // the lookup below stays in the table
     }
     if (ok && voxel_site(d, v[0], v[1], v[2]) < 0) ok = false;
     if (ok) { p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; }
