@@ -115,6 +115,9 @@ public:
     if (cnt == 0) { set_error("geometry has no fluid sites"); return FELBM_EINVAL; }
     if (cnt >= (int64_t)1 << 31) { set_error("more than 2^31 sites"); return FELBM_EUNSUPPORTED; }
     S.n = (int)cnt; S.ns = (int)align_up((size_t)cnt, 64); S.wpr = wpr;
+    if ((int64_t)g->ny * wpr >= ((int64_t)1 << 24)) {     // row offsets use 24-bit multiplies
+      set_error("box too large: ny * ceil(nx/32) must stay below 2^24"); return FELBM_EUNSUPPORTED;
+    }
     if ((uint64_t)19 * (uint64_t)S.ns * sizeof(real) >= ((uint64_t)1 << 32) || words.size() * 8 >= ((size_t)1 << 32)) {
       set_error("instance too large: 19*sites*sizeof(real) must stay below 4 GiB");
       return FELBM_EUNSUPPORTED;

@@ -281,14 +281,17 @@ __device__ __forceinline__ void bst_rec4(double c, double p, double l, double m,
 // bit a*5+j of ok says whether offset j-2 on axis a exists (always on a
 // periodic axis).  Out-of-box coordinates are still wrapped, so every word
 // load stays inside the table.
-struct Nbhd { int xw[5], yw[5], zw[5]; unsigned ok; };
+// yr and zr hold the wrapped y and z already scaled to row-word offsets.
+struct Nbhd { int xw[5], yr[5], zr[5]; unsigned ok; };
 
+// offsets are within +-2, so one correction wraps any axis of length >= 2;
+// an axis of length 1 maps everything to 0
 __device__ __forceinline__ int wrap_axis(int v, int N, int per, bool &ok)
 {
   ok = per || (v >= 0 && v < N);
-  v = v < 0 ? v + N : v; v = v < 0 ? v + N : v;
-  v = v >= N ? v - N : v; v = v >= N ? v - N : v;
-  return v;
+  v = v < 0 ? v + N : v;
+  v = v >= N ? v - N : v;
+  return N == 1 ? 0 : v;
 }
 
 template <typename real>
@@ -302,12 +305,15 @@ template <typename real>
 __device__ __forceinline__ void make_nbhd(const Inst<real> &d, int x, int y, int z, Nbhd &nb)
 {
   nb.ok = 0;
+  // row-word offsets stay below 2^32 (checked when the instance is created) and
+  // each factor below 2^24, so the full-rate 24-bit multiply is exact
+  const unsigned wpr = (unsigned)d.wpr, plane = (unsigned)d.Ny*wpr;
   #pragma unroll
   for (int j = 0; j < 5; ++j) {
     bool a, b, c;
     nb.xw[j] = wrap_axis(x + j - 2, d.Nx, d.px, a);
-    nb.yw[j] = wrap_axis(y + j - 2, d.Ny, d.py, b);
-    nb.zw[j] = wrap_axis(z + j - 2, d.Nz, d.pz, c);
+    nb.yr[j] = (int)__umul24((unsigned)wrap_axis(y + j - 2, d.Ny, d.py, b), wpr);
+    nb.zr[j] = (int)__umul24((unsigned)wrap_axis(z + j - 2, d.Nz, d.pz, c), plane);
     nb.ok |= (a ? 1u << j : 0u) | (b ? 1u << (5 + j) : 0u) | (c ? 1u << (10 + j) : 0u);
   }
 }
@@ -320,8 +326,8 @@ __device__ __forceinline__ unsigned nb_lookup(const Inst<real> &d, rsrc_t rW, co
                                               int dx, int dy, int dz, unsigned self_off,
                                               bool &fluid, bool &solid)
 {
-  const int xx = nb.xw[dx + 2], yy = nb.yw[dy + 2], zz = nb.zw[dz + 2];
-  const unsigned widx = ((unsigned)zz*(unsigned)d.Ny + (unsigned)yy)*(unsigned)d.wpr + ((unsigned)xx >> 5);
+  const int xx = nb.xw[dx + 2];
+  const unsigned widx = (unsigned)nb.zr[dz + 2] + (unsigned)nb.yr[dy + 2] + ((unsigned)xx >> 5);
   const uint64_t w = bld<uint64_t>(rW, widx*8u, 0u);
   const unsigned lo = (unsigned)w, b = (unsigned)xx & 31u;
   const unsigned idx = (unsigned)(w >> 32) + (unsigned)__popc(lo & ((1u << b) - 1u));
