@@ -129,6 +129,22 @@ def test_c1_full_2000_steps():
         assert np.abs(fo[k] - fg[k]).max() <= 1e-13, k
 
 
+def test_c2_porous_150_fp64_200_steps():
+    """The 150^3 sphere pack at porosity 0.40 (1.35e6 sites), MRT, all switches,
+    mass correction on: FP64 bit-identical to the oracle after 200 steps."""
+    w = W.workload("porous_150")
+    o, g = make_pair(w["solid"], w["params"], 1, 1, c0=w["c0"])
+    _say("C2 200 steps, n =", g.n)
+    o.run(200)
+    g.step(200)
+    ho, go = o.download_state()
+    hg, gg = g.download_state()
+    dh, dg = float(np.abs(ho - hg).max()), float(np.abs(go - gg).max())
+    _say("C2 max|dh| max|dg|", dh, dg)
+    assert dh == 0.0 and dg == 0.0
+    assert abs(o.current_mass() - g.current_mass()) <= 1e-9
+
+
 def _fp32_errors(o, g, steps):
     o.run(steps)
     g.step(steps)
