@@ -62,6 +62,13 @@ int felbm_tracers_init(felbm_engine *e, int slot, int64_t np, const double *xyz,
 { ENG(e); return e->impl->tracers_init(slot, np, xyz, dm, seed, substeps, velocity_skip); }
 int felbm_tracers_download(felbm_engine *e, int slot, double *xyz, double *uvw, int32_t *blocked)
 { ENG(e); return e->impl->tracers_download(slot, xyz, uvw, blocked); }
+int felbm_tracers_stretching(felbm_engine *e, int slot, int enable, const double *rho_hat, const double *log_rho,
+                             uint64_t orient_seed, int phase_split, int row_capacity)
+{ ENG(e); return e->impl->tracers_stretching(slot, enable, rho_hat, log_rho, orient_seed, phase_split, row_capacity); }
+int felbm_tracers_stretch_rows(felbm_engine *e, int slot, double *rows, int max_rows, int *count, int64_t *lost)
+{ ENG(e); return e->impl->tracers_stretch_rows(slot, rows, max_rows, count, lost); }
+int felbm_tracers_stretch_state(felbm_engine *e, int slot, double *rho_hat, double *log_rho)
+{ ENG(e); return e->impl->tracers_stretch_state(slot, rho_hat, log_rho); }
 int felbm_tracers_count(felbm_engine *e, int slot, int64_t *np)
 { ENG(e); if (!np) { fm::set_error("null output pointer"); return FELBM_EINVAL; } return e->impl->tracers_count(slot, np); }
 

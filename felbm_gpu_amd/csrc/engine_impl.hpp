@@ -55,6 +55,10 @@ struct Slot {
   char *tr_base = nullptr;        // pos [3np] f64, dm [np] f64, uvw [3np] f64, blocked [np] i32, tu [3ns] real
   double *tr_pos = nullptr, *tr_dm = nullptr, *tr_uvw = nullptr;
   int *tr_blocked = nullptr;
+  // Lagrangian stretching (rho_hat, log_rho, block partials, row ring)
+  char *tr_sbase = nullptr;
+  Stretch st{};
+  unsigned long long tr_drained = 0;   // tracer steps already returned as rows
 };
 
 template <typename real>
@@ -174,6 +178,7 @@ public:
     FM_HIP(hipFree(slots_[slot].base));
     if (slots_[slot].ob_base) FM_HIP(hipFree(slots_[slot].ob_base));
     if (slots_[slot].tr_base) FM_HIP(hipFree(slots_[slot].tr_base));
+    if (slots_[slot].tr_sbase) FM_HIP(hipFree(slots_[slot].tr_sbase));
     slots_[slot] = Slot<real>();
     dirty_ = true;
     return rebuild();
@@ -477,14 +482,15 @@ public:
     Slot<real> &S = slots_[slot];
     FM_HIP(hipStreamSynchronize(stream_));
     if (S.tr_base) { FM_HIP(hipFree(S.tr_base)); S.tr_base = nullptr; }
-    S.tr_np = 0; S.d.trk = 0; S.d.tu = nullptr;
+    if (S.tr_sbase) { FM_HIP(hipFree(S.tr_sbase)); S.tr_sbase = nullptr; }
+    S.tr_np = 0; S.d.trk = 0; S.d.tu = nullptr; S.d.tr_phase = 0; S.st = Stretch{}; S.tr_drained = 0;
     dirty_desc_ = true;
     if (np == 0) return FELBM_OK;
     const size_t n = (size_t)np, ns = (size_t)S.ns;
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off = align_up(off + b, 256); return o; };
     const size_t o_p = take(3 * n * 8), o_d = take(n * 8), o_u = take(3 * n * 8), o_b = take(n * 4);
-    const size_t o_t = take(3 * ns * sizeof(real));
+    const size_t o_t = take(4 * ns * sizeof(real));   // u (3) and c for the phase split
     if (hipMalloc((void **)&S.tr_base, off) != hipSuccess) { set_error("device allocation failed"); return FELBM_ENOMEM; }
     FM_HIP(hipMemsetAsync(S.tr_base, 0, off, stream_));
     S.tr_pos = (double *)(S.tr_base + o_p); S.tr_dm = (double *)(S.tr_base + o_d);
@@ -513,6 +519,90 @@ public:
     }
     if (xyz) FM_HIP(hipMemcpyAsync(xyz, S.tr_pos, 3 * n * 8, hipMemcpyDeviceToHost, stream_));
     if (blocked) FM_HIP(hipMemcpyAsync(blocked, S.tr_blocked, n * 4, hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    return FELBM_OK;
+  }
+
+  // stretching on/off for the slot's tracers; rho_hat/log_rho restore a saved
+  // state (NULL: isotropic unit vectors from orient_seed, log_rho = 0)
+  int tracers_stretching(int slot, int enable, const double *rho_hat, const double *log_rho, uint64_t orient_seed,
+                         int phase_split, int row_capacity) override {
+    if (int e = check(slot)) return e;
+    Slot<real> &S = slots_[slot];
+    if (!S.tr_np) { set_error("no tracers on this slot"); return FELBM_ESTATE; }
+    FM_HIP(hipStreamSynchronize(stream_));
+    if (S.tr_sbase) { FM_HIP(hipFree(S.tr_sbase)); S.tr_sbase = nullptr; }
+    S.st = Stretch{}; S.d.tr_phase = 0; dirty_desc_ = true;
+    if (!enable) return FELBM_OK;
+    if (row_capacity < 1) { set_error("row capacity must be positive"); return FELBM_EINVAL; }
+    const size_t n = (size_t)S.tr_np, nb = (n + 255) / 256;
+    size_t off = 0;
+    auto take = [&](size_t b) { size_t o = off; off = align_up(off + b, 256); return o; };
+    const size_t o_r = take(3 * n * 8), o_l = take(n * 8), o_p = take(nb * 8 * 8), o_w = take((size_t)row_capacity * 9 * 8);
+    if (hipMalloc((void **)&S.tr_sbase, off) != hipSuccess) { set_error("device allocation failed"); return FELBM_ENOMEM; }
+    std::vector<double> rh(3 * n), lr(n, 0.0);
+    if (rho_hat) {
+      std::copy(rho_hat, rho_hat + 3 * n, rh.begin());
+    } else {
+      // isotropic orientations: normalised triples of standard normals
+      for (size_t t = 0; t < n; ++t) {
+        double v[3], nrm = 0.0;
+        for (int a = 0; a < 3; ++a) {
+          const uint64_t r0 = mix64(orient_seed ^ mix64(6 * t + 2 * a)), r1 = mix64(orient_seed ^ mix64(6 * t + 2 * a + 1));
+          const double u0 = (double)((r0 >> 11) + 1) * 0x1p-53, v0 = (double)(r1 >> 11) * 0x1p-53;
+          v[a] = std::sqrt(-2.0 * std::log(u0)) * std::cos(6.283185307179586 * v0);
+          nrm += v[a] * v[a];
+        }
+        nrm = std::sqrt(nrm);
+        for (int a = 0; a < 3; ++a) rh[3 * t + a] = v[a] / nrm;
+      }
+    }
+    if (log_rho) std::copy(log_rho, log_rho + n, lr.begin());
+    FM_HIP(hipMemcpyAsync(S.tr_sbase + o_r, rh.data(), 3 * n * 8, hipMemcpyHostToDevice, stream_));
+    FM_HIP(hipMemcpyAsync(S.tr_sbase + o_l, lr.data(), n * 8, hipMemcpyHostToDevice, stream_));
+    FM_HIP(hipMemsetAsync(S.tr_sbase + o_p, 0, off - o_p, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    S.st.rho_hat = (double *)(S.tr_sbase + o_r); S.st.log_rho = (double *)(S.tr_sbase + o_l);
+    S.st.partials = (double *)(S.tr_sbase + o_p); S.st.rows = (double *)(S.tr_sbase + o_w); S.st.cap = row_capacity;
+    S.d.tr_phase = phase_split ? 1 : 0;
+    unsigned long long ts = 0;
+    FM_HIP(hipMemcpy(&ts, S.d.acc + ACC_FLAGS + 2, 8, hipMemcpyDeviceToHost));
+    S.tr_drained = ts;
+    return FELBM_OK;
+  }
+
+  // rows of the steps since the last call, oldest first (at most max_rows);
+  // rows older than the ring capacity are counted in *lost
+  int tracers_stretch_rows(int slot, double *rows, int max_rows, int *count, int64_t *lost) override {
+    if (int e = check(slot)) return e;
+    Slot<real> &S = slots_[slot];
+    if (!S.st.rows) { set_error("stretching is not enabled on this slot"); return FELBM_ESTATE; }
+    if (!count || max_rows < 0 || (max_rows && !rows)) { set_error("invalid row buffer"); return FELBM_EINVAL; }
+    unsigned long long ts = 0;
+    FM_HIP(hipMemcpyAsync(&ts, S.d.acc + ACC_FLAGS + 2, 8, hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    unsigned long long first = S.tr_drained, lostn = 0;
+    if (ts - first > (unsigned long long)S.st.cap) { lostn = ts - first - S.st.cap; first = ts - S.st.cap; }
+    const unsigned long long take_n = std::min<unsigned long long>(ts - first, (unsigned long long)max_rows);
+    std::vector<double> ring((size_t)S.st.cap * 9);
+    FM_HIP(hipMemcpy(ring.data(), S.st.rows, ring.size() * 8, hipMemcpyDeviceToHost));
+    for (unsigned long long r = 0; r < take_n; ++r) {
+      const size_t src = (size_t)((first + r) % (unsigned long long)S.st.cap);
+      std::copy(ring.begin() + 9 * src, ring.begin() + 9 * src + 9, rows + 9 * r);
+    }
+    S.tr_drained = first + take_n;
+    *count = (int)take_n;
+    if (lost) *lost = (int64_t)lostn;
+    return FELBM_OK;
+  }
+
+  int tracers_stretch_state(int slot, double *rho_hat, double *log_rho) override {
+    if (int e = check(slot)) return e;
+    Slot<real> &S = slots_[slot];
+    if (!S.st.rows) { set_error("stretching is not enabled on this slot"); return FELBM_ESTATE; }
+    const size_t n = (size_t)S.tr_np;
+    if (rho_hat) FM_HIP(hipMemcpyAsync(rho_hat, S.st.rho_hat, 3 * n * 8, hipMemcpyDeviceToHost, stream_));
+    if (log_rho) FM_HIP(hipMemcpyAsync(log_rho, S.st.log_rho, n * 8, hipMemcpyDeviceToHost, stream_));
     FM_HIP(hipStreamSynchronize(stream_));
     return FELBM_OK;
   }
@@ -586,6 +676,7 @@ private:
     for (auto &s : slots_) if (s.used && s.base) (void)hipFree(s.base);
     for (auto &s : slots_) if (s.used && s.ob_base) (void)hipFree(s.ob_base);
     for (auto &s : slots_) if (s.used && s.tr_base) (void)hipFree(s.tr_base);
+    for (auto &s : slots_) if (s.used && s.tr_sbase) (void)hipFree(s.tr_sbase);
     if (d_ent_all_) (void)hipFree(d_ent_all_);
     slots_.clear();
     if (d_desc_) (void)hipFree(d_desc_);
@@ -732,9 +823,14 @@ private:
     for (size_t q = 0; q < slots_.size(); ++q) {
       Slot<real> &S = slots_[q];
       if (!S.used || !S.tr_np) continue;
-      hipLaunchKernelGGL(k_tracers<real>, dim3((S.tr_np + 255) / 256), blk, 0, stream_, d_desc_, slot_desc_[q],
-                         S.tr_pos, (const double *)S.tr_dm, S.tr_blocked, S.tr_np, S.tr_seed, S.tr_substeps);
+      const int nb = (S.tr_np + 255) / 256;
+      hipLaunchKernelGGL(k_tracers<real>, dim3(nb), blk, 0, stream_, d_desc_, slot_desc_[q],
+                         S.tr_pos, (const double *)S.tr_dm, S.tr_blocked, S.tr_np, S.tr_seed, S.tr_substeps, S.st);
       FM_LAUNCH();
+      if (S.st.rows) {
+        hipLaunchKernelGGL(k_tracer_row<real>, dim3(1), dim3(64), 0, stream_, d_desc_, slot_desc_[q], S.st, nb, S.tr_np);
+        FM_LAUNCH();
+      }
     }
     if (timed) FM_HIP(hipEventRecord(ev_[3], stream_));
     for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_key(k, 4, dim3(nvar_[k]), d_var_[k], stats); FM_LAUNCH(); }

@@ -100,7 +100,8 @@ struct Inst {
   // tracers: the velocity snapshot they are advected on
   int trk;                     // tracers present
   int tr_vskip;                // refresh the snapshot every tr_vskip steps
-  real *tu;                    // [3][ns] snapshot of u (the field pass's u*)
+  int tr_phase;                // the snapshot also holds c (phase split of the stretching rows)
+  real *tu;                    // [3 or 4][ns] snapshot of u (the field pass's u*), then c
 };
 
 // site classes of the open boundary
@@ -689,6 +690,7 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
     if (ts % (unsigned long long)d.tr_vskip == 0) {
       const long ns = d.ns;
       stg(d.tu, i, ux); stg(d.tu, ns + i, uy); stg(d.tu, 2*ns + i, uz);
+      if (d.tr_phase) stg(d.tu, 3*ns + i, c);
     }
   }
 }

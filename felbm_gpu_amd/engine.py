@@ -210,6 +210,31 @@ class MultiPhaseGPU:
                                               blk.ctypes.data_as(C.POINTER(C.c_int32))))
         return xyz, uvw, blk
 
+    # particles_stretching (docs/HANDOFF_stretching.md:165-210)
+    def tracers_stretching(self, enable=True, rho_hat=None, log_rho=None, seed=20260801, phase_split=True,
+                           capacity=4096):
+        rh = None if rho_hat is None else np.ascontiguousarray(rho_hat, dtype=np.float64).reshape(-1)
+        lr = None if log_rho is None else np.ascontiguousarray(log_rho, dtype=np.float64).reshape(-1)
+        check(self.lib.felbm_tracers_stretching(self.h, self.slot, int(bool(enable)),
+                                                None if rh is None else _dptr(rh), None if lr is None else _dptr(lr),
+                                                int(seed), int(bool(phase_split)), int(capacity)))
+
+    def tracers_stretch_rows(self, max_rows=100000):
+        """take_stretch_rows: (rows [k][9], lost); columns step, lambda, mean_log_rho,
+        var_log_rho, n_active, lambda_w, lambda_nw, n_w, n_nw."""
+        buf = np.empty((int(max_rows), 9))
+        cnt, lost = C.c_int(), C.c_int64()
+        check(self.lib.felbm_tracers_stretch_rows(self.h, self.slot, _dptr(buf), int(max_rows), C.byref(cnt),
+                                                  C.byref(lost)))
+        return buf[:cnt.value].copy(), lost.value
+
+    def tracers_stretch_state(self):
+        np_ = C.c_int64()
+        check(self.lib.felbm_tracers_count(self.h, self.slot, C.byref(np_)))
+        rh, lr = np.empty((np_.value, 3)), np.empty(np_.value)
+        check(self.lib.felbm_tracers_stretch_state(self.h, self.slot, _dptr(rh), _dptr(lr)))
+        return rh, lr
+
     def run(self, t_begin, t_end, timedep="constant", period=1.0):
         """The driver's stepping loop (src/felbm_gpu_main.cu:701-749 without
         I/O): for t in [t_begin, t_end) set forcing_factor_at(t), step, apply

@@ -157,6 +157,22 @@ int  felbm_tracers_init(felbm_engine *e, int slot, int64_t np, const double *xyz
  * rejected (wall-blocked) substeps [np]; any pointer may be NULL */
 int  felbm_tracers_download(felbm_engine *e, int slot, double *xyz, double *uvw, int32_t *blocked);
 int  felbm_tracers_count(felbm_engine *e, int slot, int64_t *np);
+/* Lagrangian stretching of the slot's tracers (particles_stretching,
+ * docs/HANDOFF_stretching.md:165-210; rows as src/felbm_gpu_main.cu:643-671).
+ * enable = 0 turns it off.  rho_hat [3*np] / log_rho [np] restore a saved state
+ * (particles_<step>.h5); NULL gives isotropic unit vectors from orient_seed
+ * (particles_stretching_seed) and log_rho = 0.  phase_split also snapshots c
+ * for the wetting / non-wetting split.  One row per step is kept on the device
+ * in a ring of row_capacity rows. */
+int  felbm_tracers_stretching(felbm_engine *e, int slot, int enable, const double *rho_hat,
+                              const double *log_rho, uint64_t orient_seed, int phase_split,
+                              int row_capacity);
+/* take_stretch_rows: the rows of the steps since the last call, oldest first,
+ * 9 doubles each (step, lambda, mean_log_rho, var_log_rho, n_active, lambda_w,
+ * lambda_nw, n_w, n_nw); at most max_rows; *lost counts rows the ring dropped */
+int  felbm_tracers_stretch_rows(felbm_engine *e, int slot, double *rows, int max_rows, int *count,
+                                int64_t *lost);
+int  felbm_tracers_stretch_state(felbm_engine *e, int slot, double *rho_hat, double *log_rho);
 
 /* ---- verification helpers -------------------------------------------------
  * Reference-format tables (multiphase_gpu.cuh:93-208) regenerated on the GPU

@@ -51,9 +51,18 @@ def normals3(seed, ids, draw):
 
 
 class Tracers:
-    """Tracers on a lattice given by its solid mask [nz][ny][nx]."""
+    """Tracers on a lattice given by its solid mask [nz][ny][nx].
 
-    def __init__(self, solid, periodic, pos, dm, seed=0, substeps=1, velocity_skip=1):
+    With rho_hat given, each tracer also carries a material line element
+    (docs/HANDOFF_stretching.md:165-210, Eq. 12): per substep, with J the
+    gradient of the trilinear interpolant at the substep's start position,
+    lambda_s = rho_hat . J rho_hat; rho* = rho_hat + dt J rho_hat;
+    log_rho += log|rho*|; rho_hat = rho*/|rho*|.  A tracer not blocked during
+    the step contributes the mean of its lambda_s to the step's row and is
+    classed wetting when c < 0.5 at its nearest node at the start of the step.
+    """
+
+    def __init__(self, solid, periodic, pos, dm, seed=0, substeps=1, velocity_skip=1, rho_hat=None, log_rho=None):
         self.solid = np.asarray(solid, dtype=np.uint8)
         nz, ny, nx = self.solid.shape
         self.N = (nx, ny, nz)
@@ -67,6 +76,12 @@ class Tracers:
         self.blocked = np.zeros(len(self.dm), dtype=np.int64)
         self.step_count = 0
         self.u = None
+        self.c = None
+        self.rho_hat = None if rho_hat is None else np.array(rho_hat, dtype=np.float64).reshape(-1, 3)
+        self.log_rho = None
+        if self.rho_hat is not None:
+            self.log_rho = np.zeros(len(self.dm)) if log_rho is None else np.array(log_rho, dtype=np.float64)
+        self.rows = []
 
     def _site(self, v):
         """site index of in-box voxel coordinates (x, y, z) arrays, -1 solid"""
@@ -98,16 +113,86 @@ class Tracers:
                         out[:, a] = np.where(ok, out[:, a] + w * u[a][ss], out[:, a])
         return out
 
-    def step(self, ux, uy, uz):
-        """One LBM step of tracer motion on the step's velocity (site arrays)."""
+    def interp_uJ(self, p, u):
+        """u and J[a][b] = d u_a / d x_b of the trilinear interpolant."""
+        fl = np.floor(p)
+        o = fl.astype(np.int64)
+        f = p - fl
+        out = np.zeros_like(p)
+        J = np.zeros((len(p), 3, 3))
+        for cz in (0, 1):
+            for cy in (0, 1):
+                for cx in (0, 1):
+                    c = (cx, cy, cz)
+                    v = [o[:, a] + c[a] for a in range(3)]
+                    ok = np.ones(len(p), dtype=bool)
+                    for a in range(3):
+                        if self.per[a]:
+                            v[a] = np.mod(v[a], self.N[a])
+                        else:
+                            ok &= (v[a] >= 0) & (v[a] < self.N[a])
+                            v[a] = np.clip(v[a], 0, self.N[a] - 1)
+                    s = self._site(v)
+                    ok &= s >= 0
+                    wx = f[:, 0] if cx else 1.0 - f[:, 0]
+                    wy = f[:, 1] if cy else 1.0 - f[:, 1]
+                    wz = f[:, 2] if cz else 1.0 - f[:, 2]
+                    w = (wx * wy) * wz
+                    g = ((1.0 if cx else -1.0) * (wy * wz), (1.0 if cy else -1.0) * (wx * wz),
+                         (1.0 if cz else -1.0) * (wx * wy))
+                    ss = np.where(ok, s, 0)
+                    for a in range(3):
+                        va = u[a][ss]
+                        out[:, a] = np.where(ok, out[:, a] + w * va, out[:, a])
+                        for b in range(3):
+                            J[:, a, b] = np.where(ok, J[:, a, b] + g[b] * va, J[:, a, b])
+        return out, J
+
+    def nearest_c(self, p):
+        if self.c is None:
+            return np.full(len(p), np.nan)
+        v = []
+        ok = np.ones(len(p), dtype=bool)
+        for a in range(3):
+            va = np.rint(p[:, a]).astype(np.int64)
+            if self.per[a]:
+                va = np.mod(va, self.N[a])
+            else:
+                ok &= (va >= 0) & (va < self.N[a])
+                va = np.clip(va, 0, self.N[a] - 1)
+            v.append(va)
+        s = self._site(v)
+        ok &= s >= 0
+        return np.where(ok, self.c[np.where(ok, s, 0)], np.nan)
+
+    def step(self, ux, uy, uz, c=None):
+        """One LBM step of tracer motion on the step's velocity (site arrays);
+        c (site array) feeds the phase split of the stretching rows."""
         if self.u is None or self.step_count % self.vskip == 0:
             self.u = (np.array(ux, dtype=np.float64), np.array(uy, dtype=np.float64), np.array(uz, dtype=np.float64))
+            self.c = None if c is None else np.array(c, dtype=np.float64)
         dt = 1.0 / self.substeps
         amp = np.where(self.dm > 0.0, np.sqrt(2.0 * self.dm * dt), 0.0)
         ids = np.arange(len(self.dm), dtype=np.uint64)
         p = self.pos
+        st = self.rho_hat is not None
+        if st:
+            c0 = self.nearest_c(p)
+            rh, lr = self.rho_hat.copy(), self.log_rho.copy()
+            lam = np.zeros(len(p))
+            nblk = np.zeros(len(p), dtype=np.int64)
         for s in range(self.substeps):
-            u = self.interp(p, self.u)
+            if st:
+                u, J = self.interp_uJ(p, self.u)
+                Jr = np.stack([(J[:, a, 0] * rh[:, 0] + J[:, a, 1] * rh[:, 1]) + J[:, a, 2] * rh[:, 2]
+                               for a in range(3)], axis=1)
+                lam = lam + ((rh[:, 0] * Jr[:, 0] + rh[:, 1] * Jr[:, 1]) + rh[:, 2] * Jr[:, 2])
+                r = rh + dt * Jr
+                g = np.sqrt((r[:, 0] * r[:, 0] + r[:, 1] * r[:, 1]) + r[:, 2] * r[:, 2])
+                lr = lr + np.log(g)
+                rh = r / g[:, None]
+            else:
+                u = self.interp(p, self.u)
             q = p + u * dt
             if np.any(self.dm > 0.0):
                 xi = normals3(self.seed, ids, self.step_count * self.substeps + s)
@@ -131,5 +216,18 @@ class Tracers:
             ok &= self._site(v) >= 0
             p = np.where(ok[:, None], q, p)
             self.blocked += ~ok
+            if st:
+                nblk += ~ok
         self.pos = p
+        if st:
+            self.rho_hat, self.log_rho = rh, lr
+            act = nblk == 0
+            li = lam / self.substeps
+            wet, dry = act & (c0 < 0.5), act & (c0 >= 0.5)
+            n_act, n_w, n_nw = int(act.sum()), int(wet.sum()), int(dry.sum())
+            mean = lr.sum() / len(lr)
+            self.rows.append([self.step_count + 1, li[act].sum() / n_act if n_act else np.nan, mean,
+                              (lr * lr).sum() / len(lr) - mean * mean, n_act,
+                              li[wet].sum() / n_w if n_w else np.nan, li[dry].sum() / n_nw if n_nw else np.nan,
+                              n_w, n_nw])
         self.step_count += 1

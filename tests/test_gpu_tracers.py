@@ -76,3 +76,39 @@ def test_tracers_fp32_engine_within_tolerance():
     d = float(np.abs(xyz[det] - t.pos[det]).max())
     _say("tracers fp32 engine: max|dx| (Dm=0) %.3e" % d)
     assert d <= 1e-5
+
+
+def test_stretching_matches_restatement_fp64():
+    """Eq. (12) stretching with the phase split on the real flow: positions of
+    the Dm = 0 tracers bit-identical, rho_hat and log_rho within 1e-12 (sqrt
+    and log of two libraries), rows within 1e-10."""
+    from oracle.tracers import Tracers
+    o, g, t, dm = setup(substeps=2, vskip=1)
+    n = len(dm)
+    r0 = np.random.default_rng(4).normal(size=(n, 3))
+    r0 /= np.linalg.norm(r0, axis=1)[:, None]
+    g.tracers_stretching(True, rho_hat=r0, phase_split=True, capacity=64)
+    t.rho_hat, t.log_rho = r0.copy(), np.zeros(n)
+    for _ in range(16):
+        o.step()
+        o.apply_mass_correction()
+        t.step(o.field("ux"), o.field("uy"), o.field("uz"), o.field("c"))
+    g.step(16)
+    xyz, uvw, blk = g.tracers_download()
+    rh, lr = g.tracers_stretch_state()
+    rows, lost = g.tracers_stretch_rows()
+    det = dm == 0.0
+    _say("stretching: max|drho_hat| %.3e max|dlog_rho| %.3e rows %d"
+         % (np.abs(rh - t.rho_hat).max(), np.abs(lr - t.log_rho).max(), len(rows)))
+    assert float(np.abs(xyz[det] - t.pos[det]).max()) == 0.0
+    assert np.abs(rh - t.rho_hat).max() <= 1e-12 and np.abs(lr - t.log_rho).max() <= 1e-12
+    assert lost == 0 and len(rows) == 16
+    ref = np.array(t.rows)
+    assert np.array_equal(rows[:, 0], ref[:, 0]) and np.array_equal(rows[:, [4, 7, 8]], ref[:, [4, 7, 8]])
+    for col in (1, 2, 3, 5, 6):
+        assert np.allclose(rows[:, col], ref[:, col], rtol=1e-10, atol=1e-18), col
+    # a second drain returns nothing new; more steps than the ring holds are counted as lost
+    assert len(g.tracers_stretch_rows()[0]) == 0
+    g.step(70)
+    rows2, lost2 = g.tracers_stretch_rows()
+    assert len(rows2) == 64 and lost2 == 6 and rows2[-1, 0] == 16 + 70

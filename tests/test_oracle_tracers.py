@@ -11,7 +11,12 @@ and README.md:428-446):
   - no tracer ever ends in a solid voxel (wall blocking), and blocked moves
     are counted;
   - the normal generator is deterministic per (seed, tracer, draw) with mean 0
-    and variance 1.
+    and variance 1;
+  - stretching (Eq. 12): in a uniform shear log|rho| follows the closed form of
+    the composed shear and the first row's lambda is the substep mean of
+    G rho_x rho_y; in solid-body rotation lambda vanishes while explicit Euler
+    grows |rho| by sqrt(1 + (Omega dt)^2) per substep; the wetting and
+    non-wetting rows recombine to the total.
 """
 import numpy as np
 
@@ -106,3 +111,76 @@ def test_normals_are_deterministic_standard_normal():
         assert not np.array_equal(a[q], c[q])
         assert abs(a[q].mean()) < 0.02 and abs(a[q].var() - 1.0) < 0.02
     assert abs(np.corrcoef(a[0], a[1])[0, 1]) < 0.02
+
+
+# ---- Lagrangian stretching (docs/HANDOFF_stretching.md:165-210, :252-289) ----
+
+def _unit(n, seed):
+    v = np.random.default_rng(seed).normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1)[:, None]
+
+
+def test_stretching_uniform_shear_is_exact():
+    # u_x = G y: J = [[0, G, 0], 0, 0] exactly; S substeps of rho* = rho + dt J rho
+    # compose to one shear by G t, so log|rho| is known in closed form
+    solid = np.zeros((4, 24, 10), np.uint8)
+    G, S, steps = 2e-3, 3, 40
+    rng = np.random.default_rng(5)
+    p0 = np.column_stack([rng.uniform(0, 9, 200), rng.uniform(6, 17, 200), rng.uniform(0, 3, 200)])
+    r0 = _unit(200, 6)
+    t = Tracers(solid, (1, 0, 1), p0, np.zeros(200), substeps=S, rho_hat=r0)
+    z0 = site_u(solid, lambda x, y, z: 0.0 * x)
+    ux = site_u(solid, lambda x, y, z: G * y)
+    for _ in range(steps):
+        t.step(ux, z0, z0)
+    T = float(steps)
+    expect = 0.5 * np.log((r0[:, 0] + T * G * r0[:, 1]) ** 2 + r0[:, 1] ** 2 + r0[:, 2] ** 2)
+    assert np.abs(t.log_rho - expect).max() < 1e-13
+    assert np.abs(np.linalg.norm(t.rho_hat, axis=1) - 1.0).max() < 1e-15
+    # first row: lambda = < rho_x rho_y G > averaged over the substeps of step 1
+    lam = np.zeros(200)
+    rh = r0.copy()
+    for _ in range(S):
+        lam += G * rh[:, 0] * rh[:, 1]
+        rh = rh + (1.0 / S) * np.column_stack([G * rh[:, 1], 0 * rh[:, 1], 0 * rh[:, 1]])
+        rh /= np.linalg.norm(rh, axis=1)[:, None]
+    assert abs(t.rows[0][1] - (lam / S).mean()) < 1e-15
+    assert t.rows[0][0] == 1 and t.rows[-1][0] == steps and t.rows[0][4] == 200
+
+
+def test_stretching_rotation_has_zero_lambda_and_euler_growth():
+    # solid-body rotation: J antisymmetric, so rho.J rho vanishes; explicit Euler
+    # still inflates |rho| by sqrt(1 + (Omega dt)^2) per substep for rho in the plane
+    solid = np.zeros((3, 40, 40), np.uint8)
+    Om, S, steps = 1e-3, 2, 30
+    p0 = np.column_stack([np.full(50, 19.3), np.full(50, 20.6), np.ones(50)])
+    r0 = _unit(50, 7)
+    r0[:, 2] = 0.0
+    r0 /= np.linalg.norm(r0, axis=1)[:, None]
+    t = Tracers(solid, (0, 0, 1), p0, np.zeros(50), substeps=S, rho_hat=r0)
+    ux = site_u(solid, lambda x, y, z: -Om * (y - 20.0))
+    uy = site_u(solid, lambda x, y, z: Om * (x - 20.0))
+    z0 = site_u(solid, lambda x, y, z: 0.0 * x)
+    for _ in range(steps):
+        t.step(ux, uy, z0)
+    assert max(abs(r[1]) for r in t.rows) < 1e-15
+    expect = steps * S * 0.5 * np.log1p((Om / S) ** 2)
+    assert np.abs(t.log_rho - expect).max() < 1e-12
+
+
+def test_stretching_phase_split_recombines():
+    solid = np.zeros((3, 16, 16), np.uint8)
+    rng = np.random.default_rng(8)
+    n = 400
+    p0 = np.column_stack([rng.uniform(0, 15, n), rng.uniform(0, 15, n), rng.uniform(0, 2, n)])
+    t = Tracers(solid, (1, 1, 1), p0, np.full(n, 1e-3), seed=3, rho_hat=_unit(n, 9))
+    x, y, z = W.site_coords(solid)
+    ux = 1e-3 * np.sin(2 * np.pi * y / 16.0)
+    uy = 1e-3 * np.cos(2 * np.pi * x / 16.0)
+    c = (x < 8).astype(float)
+    for _ in range(10):
+        t.step(ux, uy, 0.0 * x, c)
+    for r in t.rows:
+        step, lam, mean, var, na, lw, lnw, nw, nnw = r
+        assert nw + nnw == na and nw > 0 and nnw > 0
+        assert abs((nw * lw + nnw * lnw) / na - lam) < 1e-15
