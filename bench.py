@@ -1,7 +1,7 @@
 """Benchmark of the Lee-Liu D3Q19 step on one MI355X.
 
 One step = one full time step of the hot path (moments, c stencil, mu
-stencil, fused MRT collision with push streaming, FP64 mass correction) over
+stencil, fused MRT collision with gather streaming, FP64 mass correction) over
 the 300^3 random-sphere pack at porosity 0.37 (about 1e7 fluid sites), FP32
 distributions with FP64 reductions -- the configuration the headline MLUPS is
 quoted on.  MLUPS = fluid sites x steps / wall seconds, as the reference's
@@ -116,7 +116,7 @@ def main():
     copy_gbs = copy_probe(1024, 10)
     roof = {"bound": "hbm", "achieved": coll["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": coll["alg_GBs"] / HBM_PEAK_GBS, "traffic": _profile_traffic(args.workload),
-            "kernel": "k_collide (fused MRT + push streaming)",
+            "kernel": "k_collide (fused MRT collision, gather streaming)",
             "alg_bytes_per_launch": ALG_VALUES["collide"] * bytes_per_val * n,
             "copy_probe_GBs": copy_gbs}
     mass_drift = abs(eng.current_mass() - eng.get_target_mass())

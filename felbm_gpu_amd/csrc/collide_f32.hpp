@@ -1,4 +1,4 @@
-// collide_f32.hpp -- the FP32 form of the fused collision + push streaming.
+// collide_f32.hpp -- the FP32 form of the fused collision (streamed by gather).
 //
 // The same physics as k_collide (kernels.hpp), re-associated where FP32
 // allows it (the FP32 engine is held to 1e-5 against the FP64 oracle, not to
@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
     Nbhd nb; make_nbhd(d, x, y, z, nb);
     unsigned vo[Q], fl, so, fl2;
     ring1<float, OPEN>(d, rW, nb, io, vo, fl, so);
-    // open-boundary node; a buffer node does not collide and pushes the
+    // open-boundary node; a buffer node does not collide and stores the
     // equilibrium the reference writes into it (k_open_bnd_buf)
     int eb = -1;
     const float *ovp = nullptr;
@@ -66,8 +66,8 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       for (int k = 0; k < Q; ++k) {
         float hv, gv;
         plain_eq<float>(k, oux, ouy, ouz, op, orho, oc, hv, gv);
-        push<float, HBB, OPEN>(hv, rH, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
-        push<float, HBB, OPEN>(gv, rGo, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+        store_post<float, HBB, OPEN>(hv, rH, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+        store_post<float, HBB, OPEN>(gv, rGo, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
       }
       if (stats) {
         double vx = (double)oux, vy = (double)ouy, vz = (double)ouz;
@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       float gk[Q];
       float jx = 0, jy = 0, jz = 0;
       #pragma unroll
-      for (int k = 0; k < Q; ++k) { gk[k] = bld<float>(rGi, io, (unsigned)k*nsb); acc_vec<float>(k, gk[k], jx, jy, jz); }
+      for (int k = 0; k < Q; ++k) { gk[k] = bld<float>(rGi, pull_off(k, vo, fl, io, nsb), pull_soff(k, nsb)); acc_vec<float>(k, gk[k], jx, jy, jz); }
       const float f_dcx = 0.5f*(e_dcx + b_cx), f_dcy = 0.5f*(e_dcy + b_cy), f_dcz = 0.5f*(e_dcz + b_cz);
       const float f_dpx = 0.5f*(e_dpx + b_px), f_dpy = 0.5f*(e_dpy + b_py), f_dpz = 0.5f*(e_dpz + b_pz);
       const float inv = 3.0f*irho;
@@ -165,8 +165,8 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       const float B0 = rho_cs2 + 0.5f*u_drho, Bc = 0.5f*drho_cs2, hmu = 0.5f*mu, hufg = 0.5f*u_force_g;
       const float gamma_u_sq = 1.5f*u_sq;
       const float mob = d.mobility;
-      auto pushk = [&](float v, rsrc_t r, int k) {
-        push<float, HBB, OPEN>(v, r, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+      auto storek = [&](float v, rsrc_t r, int k) {
+        store_post<float, HBB, OPEN>(v, r, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
       };
       float rr[Q], yv[Q];
       float relx = 0.0f;
@@ -184,14 +184,14 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
         // 1/2 (lmu + lapmu(x+e_k)); a non-site neighbour fetched lmu itself
         const float avgk = 0.5f*(lmu + (k ? lN[k] : lmu));
         const float A = K0 + a2*bdc[k] + a4*bdp[k] + a5*fp + mob*avgk;
-        pushk(G*A, rH, k);
+        storek(G*A, rH, k);
         const float cdk = cdc[k], dc2 = cdk + bdc[k];
         // BGK eq_g carries -(mu cd_c + fp)/2 (device_engine.cuh:516), MRT +(...)/2 (:598)
         const float eg = MRT ? w*p + wut*(B0 - Bc*cdk) + G*(0.5f*fp + hufg + hmu*cdk)
                              : w*p + wut*(B0 - Bc*cdk) + G*(hufg - 0.5f*fp - hmu*cdk);
         const float fg = wut*(Bc*dc2 - u_drm) + G*(hmu*dc2 + fp - u_fg_f);
         if constexpr (!MRT) {
-          pushk(gk[k] + relx*(eg - gk[k]) + fg, rGo, k);
+          storek(gk[k] + relx*(eg - gk[k]) + fg, rGo, k);
         } else {
           yv[k] = eg - gk[k];
           rr[k] = gk[k] + fg;
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
           #pragma unroll
           for (int m = 0; m < Q; ++m)
             if (!CONS_ROW[m] && EVEN_ROW[m] && MRTTAB.M[m][0] != 0.0) a0 += (float)MRTTAB.M[m][0]*xm[m];
-          pushk(rr[0] + a0, rGo, 0);
+          storek(rr[0] + a0, rGo, 0);
         }
         #pragma unroll
         for (int k = 1; k < Q; k += 2) {
@@ -232,8 +232,8 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
             if (EVEN_ROW[m]) E += (float)MRTTAB.M[m][k]*xm[m];
             else O += (float)MRTTAB.M[m][k]*xm[m];
           }
-          pushk(rr[k] + (E + O), rGo, k);
-          pushk(rr[k + 1] + (E - O), rGo, k + 1);
+          storek(rr[k] + (E + O), rGo, k);
+          storek(rr[k + 1] + (E - O), rGo, k + 1);
         }
       }
     }

@@ -197,11 +197,17 @@ public:
     const size_t n = (size_t)S.n, ns = (size_t)S.ns;
     std::vector<real> t(19 * ns, (real)0);
     unsigned cur = cur_of(S);
+    real *stage = cur ? S.d.g0 : S.d.g1;      // the dead g buffer between steps
+    const int nb = (S.n + 255) / 256;
+    // the arrays hold post-collision populations: store the state so the
+    // next step's gather reproduces it (k_unstream inverts the gather)
     for (int pass = 0; pass < 2; ++pass) {
       const double *src = pass ? g : h;
       for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) t[k * ns + i] = (real)src[k * n + i];
       real *dst = pass ? (cur ? S.d.g1 : S.d.g0) : S.d.h;
-      FM_HIP(hipMemcpyAsync(dst, t.data(), t.size() * sizeof(real), hipMemcpyHostToDevice, stream_));
+      FM_HIP(hipMemcpyAsync(stage, t.data(), t.size() * sizeof(real), hipMemcpyHostToDevice, stream_));
+      hipLaunchKernelGGL(k_unstream<real>, dim3(nb), dim3(256), 0, stream_, S.d, (const real *)stage, dst);
+      FM_LAUNCH();
       FM_HIP(hipStreamSynchronize(stream_));
     }
     FM_HIP(hipMemsetAsync(S.d.acc + ACC_FLAGS, 0, sizeof(unsigned long long), stream_));
@@ -235,7 +241,9 @@ public:
     FM_HIP(hipMemcpyAsync(t.data(), stage, t.size() * sizeof(real), hipMemcpyDeviceToHost, stream_));
     FM_HIP(hipStreamSynchronize(stream_));
     for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) h[k * n + i] = (double)t[k * ns + i];
-    FM_HIP(hipMemcpyAsync(t.data(), cur ? S.d.g1 : S.d.g0, t.size() * sizeof(real), hipMemcpyDeviceToHost, stream_));
+    hipLaunchKernelGGL(k_gather_copy<real>, dim3(nb), dim3(256), 0, stream_, S.d, (const real *)(cur ? S.d.g1 : S.d.g0), stage);
+    FM_LAUNCH();
+    FM_HIP(hipMemcpyAsync(t.data(), stage, t.size() * sizeof(real), hipMemcpyDeviceToHost, stream_));
     FM_HIP(hipStreamSynchronize(stream_));
     for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) g[k * n + i] = (double)t[k * ns + i];
     return FELBM_OK;
@@ -805,13 +813,22 @@ private:
     }
   }
 
+  // the moments pass gathers through the neighbour rings: open (non-periodic)
+  // variants need the in-box checks
+  void launch_moments(int key, dim3 grid, const int2 *map) {
+    if (key & 4) hipLaunchKernelGGL((k_moments<real, true>), grid, dim3(256), 0, stream_, d_desc_, map, T_);
+    else hipLaunchKernelGGL((k_moments<real, false>), grid, dim3(256), 0, stream_, d_desc_, map, T_);
+  }
+
   int launch_step(bool stats, bool timed = false) {
     if (stats) if (int e = zero_stats()) return e;
     dim3 blk(256);
     if (timed) FM_HIP(hipEventRecord(ev_[0], stream_));
-    hipLaunchKernelGGL(k_moments<real>, dim3(nmap_), blk, 0, stream_, d_desc_, d_map_, T_); FM_LAUNCH();
+    for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_moments(k, dim3(nvar_[k]), d_var_[k]); FM_LAUNCH(); }
     if (n_ent_all_) {
       hipLaunchKernelGGL(k_open_fields<real>, dim3((n_ent_all_ + 255) / 256), blk, 0, stream_, d_desc_, d_ent_all_, n_ent_all_, T_);
+      FM_LAUNCH();
+      hipLaunchKernelGGL(k_open_commit<real>, dim3((n_ent_all_ + 255) / 256), blk, 0, stream_, d_desc_, d_ent_all_, n_ent_all_);
       FM_LAUNCH();
     }
     if (timed) FM_HIP(hipEventRecord(ev_[1], stream_));
@@ -884,10 +901,12 @@ private:
     const int di = slot_desc_[&S - slots_.data()];
     const size_t ns = (size_t)S.ns;
     dim3 blk(256), grid(S.dmap_len);
-    hipLaunchKernelGGL(k_moments<real>, grid, blk, 0, stream_, d_desc_, S.dmap, T_); FM_LAUNCH();
+    launch_moments(variant(S), grid, S.dmap); FM_LAUNCH();
     if (S.obc && !S.entries.empty()) {
       const int ne = (int)S.entries.size();
       hipLaunchKernelGGL(k_open_fields<real>, dim3((ne + 255) / 256), blk, 0, stream_, d_desc_, d_ent_all_ + S.e0, ne, T_);
+      FM_LAUNCH();
+      hipLaunchKernelGGL(k_open_commit<real>, dim3((ne + 255) / 256), blk, 0, stream_, d_desc_, d_ent_all_ + S.e0, ne);
       FM_LAUNCH();
     }
     launch_key(variant(S), 2, grid, S.dmap, false, S.d.craw + export_off_ * ns, di); FM_LAUNCH();

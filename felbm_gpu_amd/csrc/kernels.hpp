@@ -1,11 +1,11 @@
 // kernels.hpp -- gfx950 kernels of the Lee-Liu D3Q19 step.
 //
 // One step is four launches over the flattened site list of every instance:
-//   k_moments   own-site h,g -> c_raw, p0, j (+ exact FP64 mass partials)
+//   k_moments   gathered (streamed) h,g -> c_raw, p0, j (+ exact FP64 mass partials)
 //   k_cfield    c stencil (r=1) -> corrected c, mu, p*   (+ optional export)
 //   k_mufield   mu stencil (r=1) -> lap(mu)
-//   k_collide   equilibria + Guo force + BGK/MRT collision, push-streamed
-//               into the next g buffer and the h buffer
+//   k_collide   equilibria + Guo force + BGK/MRT collision; post-collision
+//               populations stored at their own site (the next g buffer, h)
 // replacing the reference's 15-17 launches (multiphase_gpu.cuh:522-631).
 //
 // Layout: sites are the fluid voxels in x-fastest lexicographic order; every
@@ -16,11 +16,13 @@
 // reference's int32 stencil tables (cdm/cdp/bcase/bc1/bc2/avgnext/lcase/lc1/
 // lc2, multiphase_gpu.cuh:93-193) and the streaming op program (:209-283).
 //
-// Streaming is push: the collide kernel writes post-collision population k of
-// site x into slot k of x+e_k, or into slot opp(k) of x (halfway bounce-back)
-// when x+e_k is a solid voxel, or zeroes slot opp(k) of x when x+e_k is
-// outside a non-periodic box face.  The arrays therefore always hold the
-// post-stream state, exactly what the reference's d_h/d_g hold between steps.
+// Streaming is a gather (pull): the collide kernel stores post-collision
+// population k of site x in its own slot k (contiguous stores), as zero when it
+// would leave a non-periodic box face or hit a solid without halfway
+// bounce-back; the next moments and collide passes read population k of x from
+// slot k of x - e_k, or -- x - e_k not a site -- from slot opp(k) of x (the
+// bounced-back population).  State uploads and downloads convert to and from
+// the post-stream layout the reference's d_h/d_g hold between steps.
 //
 // FP64 arithmetic follows the reference's expression order term by term (the
 // library is built with -ffp-contract=off for the FP64 instantiation), so the
@@ -107,7 +109,7 @@ struct Inst {
 // site classes of the open boundary
 constexpr int ENT_BCPROC = 1;     // fields imposed: no velocity/pressure correction
 constexpr int ENT_EQ = 2;         // g replaced by the equilibrium of the site's fields
-constexpr int ENT_NOSTREAM = 4;   // buffer node: no collision, pushes its equilibrium
+constexpr int ENT_NOSTREAM = 4;   // buffer node: no collision, stores its equilibrium
 
 // one open-boundary node: `site` takes the imposed fields of `ref` (itself for
 // inlet/outlet nodes, the mirrored vertex for a buffer node); rkind is the
@@ -371,6 +373,27 @@ __device__ __forceinline__ void ring2(const Inst<real> &d, rsrc_t rW, const Nbhd
   }
 }
 
+// ---- pull streaming ---------------------------------------------------------------
+// The arrays hold post-collision populations at their own site (slot k = the
+// population leaving along e_k; the collision writes it as 0 when it would
+// leave the box, or hit a solid without halfway bounce-back).  Streaming is a
+// gather: the post-stream population k of x is slot k of x - e_k = x + e_opp(k)
+// when that is a site, else slot opp(k) of x itself -- the bounced (or zero)
+// population.  Opposite slots are adjacent (felbm order pairs 1-2, 3-4, ...),
+// so the scalar offset is min(k, opp k)*ns and the lane offset carries the rest.
+__device__ __forceinline__ unsigned pull_off(int k, const unsigned (&vo)[Q], unsigned fl, unsigned io, unsigned nsb)
+{
+  const int o = OPP[k];
+  const unsigned bulk = (k > o) ? vo[o] + nsb : vo[o];
+  const unsigned bb = (o > k) ? io + nsb : io;
+  return ((fl >> o) & 1u) ? bulk : bb;
+}
+__device__ __forceinline__ unsigned pull_soff(int k, unsigned nsb)
+{
+  const int o = OPP[k];
+  return (unsigned)(k < o ? k : o)*nsb;
+}
+
 // site codes for table export: >= 0 site, -1 solid voxel, -2 outside the box
 template <typename real>
 __device__ __forceinline__ int nb_site(const Inst<real> &d, const Nbhd &nb, int dx, int dy, int dz)
@@ -431,7 +454,7 @@ __device__ __forceinline__ void acc_vec(int m, real t, real &sx, real &sy, real 
 }
 
 // ==== K1: moments (k_moments, device_engine.cuh:116-135, pass 1 only) ===========
-template <typename real>
+template <typename real, bool OPEN>
 __global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ D,
                                                  const int2 *__restrict__ map, unsigned T)
 {
@@ -451,10 +474,17 @@ __global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ 
   Fx m = {0, 0, 0}, w = {0, 0, 0};
   if (i < d.n) {
     const unsigned io = (unsigned)i*RB;
+    const rsrc_t rW = make_rsrc(d.words, d.nwords*8u);
+    int x, y, z; decode(d, i, x, y, z);
+    Nbhd nb; make_nbhd(d, x, y, z, nb);
+    unsigned vo[Q], fl, so;
+    ring1<real, OPEN>(d, rW, nb, io, vo, fl, so);
+    (void)so;
     real c = 0, p = 0, ux = 0, uy = 0, uz = 0;
     #pragma unroll
-    for (int k = 0; k < Q; ++k) {
-      real hv = bld_nt<real>(rH, io, (unsigned)k*nsb), gv = bld_nt<real>(rG, io, (unsigned)k*nsb);
+    for (int k = 0; k < Q; ++k) {           // streaming (gather) and moments
+      const unsigned off = pull_off(k, vo, fl, io, nsb), soff = pull_soff(k, nsb);
+      real hv = bld<real>(rH, off, soff), gv = bld<real>(rG, off, soff);
       c += hv; p += gv;
       acc_vec<real>(k, gv, ux, uy, uz);
     }
@@ -508,10 +538,10 @@ __device__ inline double inlet_gate(const Inst<real> &d, unsigned step, double c
 
 // One thread per open-boundary node, after the moments pass.  The fields the
 // reference imposes (c, rho, p, u) are computed from the mirrored vertex's
-// distributions, stored in ov, and c is written into the c_raw field so every
-// stencil of the step reads the imposed value; the node's own raw c is kept for
-// its bulk chemical potential and relaxation rate, which the reference leaves
-// as computed from the distributions.
+// moments and stored in ov; k_open_commit then writes c into the c_raw field,
+// so every stencil of the step reads the imposed value, while the node's own
+// raw c (kept in ov) sets its bulk chemical potential and relaxation rate, which
+// the reference leaves as computed from the distributions.
 template <typename real>
 __global__ void __launch_bounds__(256) k_open_fields(const Inst<real> *__restrict__ D,
                                                      const ObEntry *__restrict__ E, int ne, unsigned T)
@@ -520,17 +550,11 @@ __global__ void __launch_bounds__(256) k_open_fields(const Inst<real> *__restric
   if (j >= ne) return;
   const ObEntry e = E[j];
   const Inst<real> &d = D[e.desc];
-  const unsigned cur = (T ^ (unsigned)d.par0) & 1u;
-  const real *g = cur ? d.g1 : d.g0;
+  (void)T;
   const long ns = d.ns;
-  // k_moments at the mirrored vertex (device_engine.cuh:116-135)
-  real c = 0, p = 0, ux = 0, uy = 0, uz = 0;
-  #pragma unroll
-  for (int k = 0; k < Q; ++k) {
-    const real hv = ldg(d.h, k*ns + e.ref), gv = ldg(g, k*ns + e.ref);
-    c += hv; p += gv;
-    acc_vec<real>(k, gv, ux, uy, uz);
-  }
+  // the moments pass's sums at the mirrored vertex (k_moments, device_engine.cuh:116-135)
+  real c = ldg(d.craw, (long)e.ref), p = ldg(d.p0, (long)e.ref);
+  real ux = ldg(d.craw, 2*ns + e.ref), uy = ldg(d.craw, 3*ns + e.ref), uz = ldg(d.craw, 4*ns + e.ref);
   const real cr = c;
   real rho = d.rho1 + d.drho*c;
   const real inv = (real)1/((real)(1.0/3.0)*rho);
@@ -549,15 +573,21 @@ __global__ void __launch_bounds__(256) k_open_fields(const Inst<real> *__restric
     if (d.use_out_prs) p = (real)d.p_out;
     if (d.use_out_fluid) { c = (real)d.c_out_fixed; rho = (real)d.rho_out_fixed; }
   }
-  real craw_own = cr;
-  if (e.site != e.ref) {
-    craw_own = 0;
-    #pragma unroll
-    for (int k = 0; k < Q; ++k) craw_own += ldg(d.h, k*ns + e.site);
-  }
+  const real craw_own = (e.site != e.ref) ? ldg(d.craw, (long)e.site) : cr;
   real *o = d.ov + 8*(long)e.eid;
   stg(o, 0, c); stg(o, 1, rho); stg(o, 2, p); stg(o, 3, ux); stg(o, 4, uy); stg(o, 5, uz); stg(o, 6, craw_own);
-  stg(d.craw, (long)e.site, c);
+}
+
+// the imposed c into c_raw, once every node has read its vertex's raw moments
+template <typename real>
+__global__ void __launch_bounds__(256) k_open_commit(const Inst<real> *__restrict__ D,
+                                                     const ObEntry *__restrict__ E, int ne)
+{
+  const int j = blockIdx.x*blockDim.x + (int)threadIdx.x;
+  if (j >= ne) return;
+  const ObEntry e = E[j];
+  const Inst<real> &d = D[e.desc];
+  stg(d.craw, (long)e.site, ldg(d.ov, 8*(long)e.eid));
 }
 
 // equilibrium the open boundary writes (k_open_bnd / k_open_bnd_buf,
@@ -723,26 +753,23 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
   bst_rec4(bld<real>(rF, io, F_C*nsb), bld<real>(rF, io, F_PS*nsb), s, m0, rF, io, F_REC*nsb);
 }
 
-// Push one post-collision population k of the site at byte offset io: into
-// slot k of x+e_k (offset vo), or -- halfway bounce-back -- into slot opp(k)
-// of x, or zero into that slot when x+e_k is outside the box.  The store's
-// scalar offset is min(k, opp k)*ns; the lane part carries the rest.
+// Store one post-collision population k of the site at byte offset io into
+// its own slot k (contiguous across the wave); the next gather reads it from
+// x + e_k, or -- neighbour not a site -- from slot opp(k) of x as the
+// bounced-back population (halfway bounce-back off a solid) or zero (outside
+// the box, or no bounce-back).  vo is unused: the gather side resolves it.
 template <typename real, bool HBB, bool OPEN>
-__device__ __forceinline__ void push(real v, rsrc_t r, int k, unsigned vo, bool fluid, bool solid,
+__device__ __forceinline__ void store_post(real v, rsrc_t r, int k, unsigned vo, bool fluid, bool solid,
                                      unsigned io, unsigned nsb)
 {
-  const int o = OPP[k];
-  const unsigned so = (unsigned)(k < o ? k : o)*nsb;
-  const unsigned bulk_off = (k > o) ? vo + nsb : vo;
-  const unsigned bb_off = (o > k) ? io + nsb : io;
-  const unsigned off = fluid ? bulk_off : bb_off;
+  (void)vo;
   real val = v;
   if constexpr (OPEN || !HBB) val = (fluid || (HBB && solid)) ? v : (real)0;
   (void)solid;
-  bst(val, r, off, so);
+  bst(val, r, io, (unsigned)k*nsb);
 }
 
-// ==== K4: fused collision + push streaming (FP64, reference order) ==================
+// ==== K4: fused collision, post-collision store (FP64, reference order) ==============
 // k_collide_fused_bgk (device_engine.cuh:459-545) and k_collide_fused_mrt
 // (:547-651), with the field quantities of the reference's separate kernels
 // (k_moments u, k_grad_cd/bd/md, k_vel_press_corr) recomputed in registers.
@@ -774,7 +801,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
     unsigned vo[Q], fl, so, fl2;
     ring1<real, OPEN>(d, rW, nb, io, vo, fl, so);
     // open-boundary node (ENT_* bits); a buffer node does not collide and
-    // pushes the equilibrium the reference writes into it (k_open_bnd_buf)
+    // stores the equilibrium the reference writes into it (k_open_bnd_buf)
     int eb = -1;
     const real *ovp = nullptr;
     if constexpr (OPEN) {
@@ -788,8 +815,8 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
       for (int k = 0; k < Q; ++k) {
         real hv, gv;
         plain_eq<real>(k, oux, ouy, ouz, op, orho, oc, hv, gv);
-        push<real, HBB, OPEN>(hv, rH, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
-        push<real, HBB, OPEN>(gv, rGo, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+        store_post<real, HBB, OPEN>(hv, rH, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+        store_post<real, HBB, OPEN>(gv, rGo, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
       }
       if (stats) {
         double vx = (double)oux, vy = (double)ouy, vz = (double)ouz;
@@ -821,7 +848,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
       real gk[Q];
       real jx = 0, jy = 0, jz = 0;
       #pragma unroll
-      for (int k = 0; k < Q; ++k) { gk[k] = bld<real>(rGi, io, (unsigned)k*nsb); acc_vec<real>(k, gk[k], jx, jy, jz); }
+      for (int k = 0; k < Q; ++k) { gk[k] = bld<real>(rGi, pull_off(k, vo, fl, io, nsb), pull_soff(k, nsb)); acc_vec<real>(k, gk[k], jx, jy, jz); }
       // per-direction central and biased values, and the vector gradients in
       // the reference's direction order
       real cdc[Q], cdp[Q], bdc[Q], bdp[Q];
@@ -882,8 +909,8 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
         if (CZ[k] > 0) s += az; else if (CZ[k] < 0) s -= az;
         return s;
       };
-      auto pushk = [&](real v, rsrc_t r, int k) {
-        push<real, HBB, OPEN>(v, r, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+      auto storek = [&](real v, rsrc_t r, int k) {
+        store_post<real, HBB, OPEN>(v, r, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
       };
       if constexpr (!MRT) {
         const real drcx = drho*e_dcx*CS2, drcy = drho*e_dcy*CS2, drcz = drho*e_dcz*CS2;
@@ -924,8 +951,8 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
           real fh = G*(dc_dk - c_over*(dp_dk - ffk) - u_fh_f + mob*avg_of(k));
           real fg = (drho_dk2 - u_drm)*dGk + (ffk - u_fg_f)*G;
           real gnew = gk[k] + relax*(eg - gk[k]) + fg;
-          pushk(eh + fh, rH, k);
-          pushk(gnew, rGo, k);
+          storek(eh + fh, rH, k);
+          storek(gnew, rGo, k);
         }
       } else {
         const real drho_cs2 = drho*CS2, rho_cs2 = rho*CS2, c_over = c/rho_cs2;
@@ -969,7 +996,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
           real ffk = mu*dc_dk + f_proj;
           real fh = G*(dc_dk - c_over*(dp_dk - ffk) - u_fh_f + mob*avg_of(k));
           fgs[k] = (drho_dk2 - u_drm)*dGk + (ffk - u_fg_f)*G;
-          pushk(eh + fh, rH, k);
+          storek(eh + fh, rH, k);
           const real yv = eg - gk[k];
           #pragma unroll
           for (int m = 1; m < Q; ++m) {
@@ -988,7 +1015,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
             const bool cons = (m == 0 || m == 3 || m == 5 || m == 7);
             if (MRTTAB.MI[k][m] != 0.0 && !cons) acc += (real)MRTTAB.MI[k][m]*xx[m];
           }
-          pushk(gk[k] + acc + fgs[k], rGo, k);
+          storek(gk[k] + acc + fgs[k], rGo, k);
         }
       }
     }
@@ -1010,6 +1037,21 @@ namespace fm {
 
 // mass sums of the current state (k_mass_weight, device_engine.cuh:1109-1126);
 // with apply, the sum of the folded-corrected c instead (the effective mass).
+// post-stream populations of site i gathered from the post-collision array a
+template <typename real>
+__device__ inline void gather_site(const Inst<real> &d, const real *a, int i, real (&v)[Q])
+{
+  constexpr unsigned RB = sizeof(real);
+  const unsigned nsb = (unsigned)d.ns*RB, io = (unsigned)i*RB;
+  const rsrc_t rW = make_rsrc(d.words, d.nwords*8u), rA = make_rsrc(a, 19u*nsb);
+  int x, y, z; decode(d, i, x, y, z);
+  Nbhd nb; make_nbhd(d, x, y, z, nb);
+  unsigned vo[Q], fl, so;
+  ring1<real, true>(d, rW, nb, io, vo, fl, so);
+  #pragma unroll
+  for (int k = 0; k < Q; ++k) v[k] = bld<real>(rA, pull_off(k, vo, fl, io, nsb), pull_soff(k, nsb));
+}
+
 template <typename real>
 __global__ void __launch_bounds__(256) k_mass_single(Inst<real> d, unsigned cur, int apply, double lambda)
 {
@@ -1017,9 +1059,11 @@ __global__ void __launch_bounds__(256) k_mass_single(Inst<real> d, unsigned cur,
   const real *g = cur ? d.g1 : d.g0; (void)g;
   Fx m = {0, 0, 0}, w = {0, 0, 0};
   if (i < d.n) {
+    real hv[Q];
+    gather_site(d, d.h, i, hv);
     real c = 0;
     #pragma unroll
-    for (int k = 0; k < Q; ++k) c += d.h[k*d.ns + i];
+    for (int k = 0; k < Q; ++k) c += hv[k];
     if (apply) c = corrected<real>(c, true, lambda);
     double cm = (double)c;
     double phi = cm*(1.0 - cm); if (phi < 0.0) phi = 0.0;
@@ -1037,17 +1081,55 @@ __global__ void __launch_bounds__(256) k_inject_copy(Inst<real> d, int apply, do
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= d.n) return;
   const int ns = d.ns;
+  real hv[Q];
+  gather_site(d, d.h, i, hv);
   real c = 0;
   #pragma unroll
-  for (int k = 0; k < Q; ++k) c += d.h[k*ns + i];
+  for (int k = 0; k < Q; ++k) c += hv[k];
   double cm = (double)c, phi = cm*(1.0 - cm);
   bool inj = apply && phi > 0.0;
   double delta_c = -lambda*phi;
   #pragma unroll
   for (int k = 0; k < Q; ++k) {
-    real v = d.h[k*ns + i];
+    real v = hv[k];
     if (inj) v += (real)(WD[k]*delta_c);
     out[k*ns + i] = v;
+  }
+}
+
+// post-stream copy of a post-collision array (state download)
+template <typename real>
+__global__ void __launch_bounds__(256) k_gather_copy(Inst<real> d, const real *a, real *out)
+{
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  real v[Q];
+  gather_site(d, a, i, v);
+  #pragma unroll
+  for (int k = 0; k < Q; ++k) out[(long)k*d.ns + i] = v[k];
+}
+
+// inverse of the gather (state upload): slot k of x holds the post-stream
+// population k of x + e_k when that is a site, else post-stream opp(k) of x
+template <typename real>
+__global__ void __launch_bounds__(256) k_unstream(Inst<real> d, const real *in, real *out)
+{
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  constexpr unsigned RB = sizeof(real);
+  const unsigned nsb = (unsigned)d.ns*RB, io = (unsigned)i*RB;
+  const rsrc_t rW = make_rsrc(d.words, d.nwords*8u), rI = make_rsrc(in, 19u*nsb), rO = make_rsrc(out, 19u*nsb);
+  int x, y, z; decode(d, i, x, y, z);
+  Nbhd nb; make_nbhd(d, x, y, z, nb);
+  unsigned vo[Q], fl, so;
+  ring1<real, true>(d, rW, nb, io, vo, fl, so);
+  #pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    const int o = OPP[k];
+    const bool f = (fl >> k) & 1u;
+    // slot k of x + e_k, or slot o of x; the scalar part min(k, o)*ns is uniform
+    const unsigned off = f ? vo[k] + (k > o ? nsb : 0u) : io + (o > k ? nsb : 0u);
+    bst(bld<real>(rI, off, pull_soff(k, nsb)), rO, io, (unsigned)k*nsb);
   }
 }
 
