@@ -71,6 +71,7 @@ SIGNATURES = {
     "felbm_tracers_init": (_I, [_P, _I, C.c_int64, _PD, _PD, C.c_uint64, _I, _I]),
     "felbm_tracers_download": (_I, [_P, _I, _PD, _PD, _PI32]),
     "felbm_tracers_count": (_I, [_P, _I, C.POINTER(C.c_int64)]),
+    "felbm_tracers_set_step": (_I, [_P, _I, C.c_uint64]),
     "felbm_tracers_stretching": (_I, [_P, _I, _I, _PD, _PD, C.c_uint64, _I, _I]),
     "felbm_tracers_stretch_rows": (_I, [_P, _I, _PD, _I, C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
     "felbm_tracers_stretch_state": (_I, [_P, _I, _PD, _PD]),

@@ -13,6 +13,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfelbm_mi355x.so")
+H5LIB = os.path.join(HERE, "libfelbm_h5.so")
+# HDF5 1.10 C library for the output/restart files (libfelbm_h5.so only)
+HDF5_DIR = os.environ.get("FELBM_HDF5_DIR", "/opt/conda")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "--offload-arch=gfx950"
@@ -52,7 +55,23 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_h5(verbose)
     return LIB
+
+
+def build_h5(verbose=False):
+    """libfelbm_h5.so: the HDF5 writers, linked against HDF5_DIR's libhdf5."""
+    if not os.path.exists(os.path.join(HDF5_DIR, "include", "hdf5.h")):
+        sys.stderr.write("HDF5 headers not found under %s: libfelbm_h5.so not built\n" % HDF5_DIR)
+        return None
+    tmp = H5LIB + ".tmp"
+    _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-Wall", "-I" + os.path.join(ROOT, "include"),
+          "-I" + os.path.join(HDF5_DIR, "include"), "-o", tmp, os.path.join(CSRC, "h5io.c"),
+          "-L" + os.path.join(HDF5_DIR, "lib"), "-lhdf5", "-Wl,-rpath," + os.path.join(HDF5_DIR, "lib")])
+    os.replace(tmp, H5LIB)
+    if verbose:
+        print("built", H5LIB)
+    return H5LIB
 
 
 if __name__ == "__main__":

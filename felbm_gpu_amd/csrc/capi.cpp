@@ -71,6 +71,8 @@ int felbm_tracers_stretch_state(felbm_engine *e, int slot, double *rho_hat, doub
 { ENG(e); return e->impl->tracers_stretch_state(slot, rho_hat, log_rho); }
 int felbm_tracers_count(felbm_engine *e, int slot, int64_t *np)
 { ENG(e); if (!np) { fm::set_error("null output pointer"); return FELBM_EINVAL; } return e->impl->tracers_count(slot, np); }
+int felbm_tracers_set_step(felbm_engine *e, int slot, uint64_t step)
+{ ENG(e); return e->impl->tracers_set_step(slot, step); }
 
 // forcing_factor_at of the reference driver (src/felbm_gpu_main.cu:587-594):
 // the time-dependence name is matched as a substring, anything else is constant

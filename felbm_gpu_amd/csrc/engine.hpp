@@ -39,6 +39,7 @@ struct EngineBase {
                            int vskip) = 0;
   virtual int tracers_download(int slot, double *xyz, double *uvw, int32_t *blocked) = 0;
   virtual int tracers_count(int slot, int64_t *np) = 0;
+  virtual int tracers_set_step(int slot, uint64_t step) = 0;
   virtual int tracers_stretching(int slot, int enable, const double *rho_hat, const double *log_rho,
                                  uint64_t orient_seed, int phase_split, int row_capacity) = 0;
   virtual int tracers_stretch_rows(int slot, double *rows, int max_rows, int *count, int64_t *lost) = 0;

@@ -621,6 +621,19 @@ public:
     return FELBM_OK;
   }
 
+  // tracer step counter: the Brownian draw index, the velocity-refresh phase
+  // and the step column of the stretching rows all follow it
+  int tracers_set_step(int slot, uint64_t step) override {
+    if (int e = check(slot)) return e;
+    Slot<real> &S = slots_[slot];
+    if (!S.tr_np) { set_error("no tracers on this slot"); return FELBM_ESTATE; }
+    const unsigned long long v = step;
+    FM_HIP(hipMemcpyAsync(S.d.acc + ACC_FLAGS + 2, &v, sizeof(v), hipMemcpyHostToDevice, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    S.tr_drained = v;
+    return FELBM_OK;
+  }
+
   int kernel_times(int nsteps, float *ms4) override {
     if (int e = flush()) return e;
     if (nmap_ == 0 || nsteps <= 0) { for (int q = 0; q < 4; ++q) ms4[q] = 0; return FELBM_OK; }

@@ -120,6 +120,9 @@ class MultiPhaseGPU:
         self.n = 0
         self.Vn = 0
         self.correct_mass = False
+        self.solid = None
+        self._tr = None          # host copy of the tracer set-up (dm, plane_id) for particle files
+        self._stretch = False
 
     # multiphase_gpu.cuh:75-351
     def init(self, solid, params, switches, periodic=(1, 1, 1), use_halfway_bb=True):
@@ -127,6 +130,7 @@ class MultiPhaseGPU:
         self.n = self.batch.n_sites(self.slot)
         self.Vn = 19 * self.n
         self.correct_mass = bool(switches.get("correct_op_mass", False))
+        self.solid = np.asarray(solid)
         return self
 
     @property
@@ -193,12 +197,21 @@ class MultiPhaseGPU:
                                           int(bool(copy_to_buffers))))
 
     # tracers: ParticleManager (felbm_gpu_main.cu:426-438, :752-777)
-    def tracers_init(self, xyz, dm, seed=0, substeps=1, velocity_skip=1):
+    def tracers_init(self, xyz, dm, seed=0, substeps=1, velocity_skip=1, plane_id=None):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         dm = np.ascontiguousarray(dm, dtype=np.float64).reshape(-1)
         assert len(dm) == len(xyz)
         check(self.lib.felbm_tracers_init(self.h, self.slot, len(dm), _dptr(xyz), _dptr(dm), int(seed),
                                           int(substeps), int(velocity_skip)))
+        self._stretch = False
+        self._tr = None if len(dm) == 0 else {
+            "dm": dm.copy(), "plane_id": None if plane_id is None else np.asarray(plane_id, dtype=np.int32).copy()}
+
+    def has_tracers(self):
+        return self._tr is not None
+
+    def tracers_set_step(self, step):
+        check(self.lib.felbm_tracers_set_step(self.h, self.slot, int(step)))
 
     def tracers_download(self):
         np_ = C.c_int64()
@@ -218,6 +231,7 @@ class MultiPhaseGPU:
         check(self.lib.felbm_tracers_stretching(self.h, self.slot, int(bool(enable)),
                                                 None if rh is None else _dptr(rh), None if lr is None else _dptr(lr),
                                                 int(seed), int(bool(phase_split)), int(capacity)))
+        self._stretch = bool(enable)
 
     def tracers_stretch_rows(self, max_rows=100000):
         """take_stretch_rows: (rows [k][9], lost); columns step, lambda, mean_log_rho,
@@ -235,14 +249,66 @@ class MultiPhaseGPU:
         check(self.lib.felbm_tracers_stretch_state(self.h, self.slot, _dptr(rh), _dptr(lr)))
         return rh, lr
 
-    def run(self, t_begin, t_end, timedep="constant", period=1.0):
-        """The driver's stepping loop (src/felbm_gpu_main.cu:701-749 without
-        I/O): for t in [t_begin, t_end) set forcing_factor_at(t), step, apply
-        the mass correction."""
-        for t in range(int(t_begin), int(t_end)):
-            self.set_forcing_factor(forcing_factor_at(timedep, period, t))
-            self.step()
-            self.apply_mass_correction()
+    def run(self, t_begin, t_end, timedep="constant", period=1.0, output=None):
+        """The driver's stepping loop (src/felbm_gpu_main.cu:701-749): for t in
+        [t_begin, t_end) set forcing_factor_at(t), step, apply the mass
+        correction.  With an ``h5io.OutputSchedule`` its files for step t are
+        written before that step, and for t_end after the last one."""
+        for t in range(int(t_begin), int(t_end) + (1 if output is not None else 0)):
+            if output is not None:
+                output.before_step(self, t)
+            if t < int(t_end):
+                self.set_forcing_factor(forcing_factor_at(timedep, period, t))
+                self.step()
+                self.apply_mass_correction()
+
+    # ---- HDF5 files (src/felbm_gpu_main.cu:47-189, :276, :395-410, :566-575, :706-770)
+    def write_fields_h5(self, path, deflate=0, f32=False):
+        """<output_name>_<t>.h5 of the current state (download_raw + write_fields_h5)."""
+        from . import h5io
+        h5io.write_fields(path, self.download(), deflate, f32)
+
+    def write_geometry_h5(self, path, deflate=0):
+        from . import h5io
+        h5io.write_geometry(path, h5io.geometry_coords(self.solid), deflate)
+
+    def write_checkpoint(self, path, step):
+        from . import h5io
+        h, g = self.download_state()
+        h5io.write_checkpoint(path, h, g, step, self.get_target_mass())
+
+    def read_checkpoint(self, path):
+        """Restores h, g and the original target mass; returns the step."""
+        from . import h5io
+        h, g, step, tm = h5io.read_checkpoint(path, self.Vn)
+        self.upload_state(h, g)
+        self.set_target_mass(tm)
+        return step
+
+    def write_particles_h5(self, path):
+        from . import h5io
+        if self._tr is None:
+            raise RuntimeError("no tracers on this lattice")
+        xyz, uvw, _ = self.tracers_download()
+        rh, lr = self.tracers_stretch_state() if self._stretch else (None, None)
+        h5io.write_particles(path, xyz, uvw, None, self._tr["dm"], self._tr["plane_id"], rh, lr)
+
+    def restore_particles_h5(self, path, step, seed=0, substeps=1, velocity_skip=1, dm=None, phase_split=True,
+                             capacity=4096):
+        """Tracers from particles_<step>.h5, in id order: positions, Dm (the
+        file's, else ``dm``), plane_id, and -- when the file holds them --
+        rho_hat/log_rho with stretching re-enabled.  The tracer step counter is
+        set to ``step``."""
+        from . import h5io
+        f = h5io.read_particles(path)
+        order = np.argsort(f["id"], kind="stable") if "id" in f else np.arange(len(f["position"]))
+        dmv = f["Dm"][order] if "Dm" in f else np.asarray(dm, dtype=np.float64)
+        pl = f["plane_id"][order] if "plane_id" in f else None
+        self.tracers_init(f["position"][order], dmv, seed, substeps, velocity_skip, plane_id=pl)
+        self.tracers_set_step(step)
+        if "rho_hat" in f:
+            self.tracers_stretching(True, f["rho_hat"][order], f["log_rho"][order], phase_split=phase_split,
+                                    capacity=capacity)
 
     def record_target_mass(self):
         check(self.lib.felbm_record_target_mass(self.h, self.slot))

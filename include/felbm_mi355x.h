@@ -157,6 +157,13 @@ int  felbm_tracers_init(felbm_engine *e, int slot, int64_t np, const double *xyz
  * rejected (wall-blocked) substeps [np]; any pointer may be NULL */
 int  felbm_tracers_download(felbm_engine *e, int slot, double *xyz, double *uvw, int32_t *blocked);
 int  felbm_tracers_count(felbm_engine *e, int slot, int64_t *np);
+/* Sets the slot's tracer step counter (0 after felbm_tracers_init), which
+ * numbers the Brownian draws, phases the particles_velocity_skip refresh and
+ * fills the step column of the stretching rows.  A restart from
+ * particles_<step>.h5 (felbm_gpu_main.cu:566-575) sets it to the checkpoint
+ * step, so the Dm > 0 tracers continue their random stream; the reference's
+ * ParticleManager re-seeds instead (docs/HANDOFF_stretching.md:543-549). */
+int  felbm_tracers_set_step(felbm_engine *e, int slot, uint64_t step);
 /* Lagrangian stretching of the slot's tracers (particles_stretching,
  * docs/HANDOFF_stretching.md:165-210; rows as src/felbm_gpu_main.cu:643-671).
  * enable = 0 turns it off.  rho_hat [3*np] / log_rho [np] restore a saved state

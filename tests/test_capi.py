@@ -14,10 +14,10 @@ from felbm_gpu_amd import workloads as W
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
+def declared_symbols(header="felbm_mi355x.h"):
     names = set()
     for f in os.listdir(os.path.join(ROOT, "include")):
-        if f.endswith(".h"):
+        if f == header:
             text = open(os.path.join(ROOT, "include", f)).read()
             text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
             names |= set(re.findall(r"\b(felbm_[a-z0-9_]+)\s*\(", text))
@@ -31,6 +31,17 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(_lib.SIGNATURES) == names
+
+
+def test_h5_library_exports_every_declared_symbol():
+    """libfelbm_h5.so (the HDF5 files) against include/felbm_h5.h."""
+    from felbm_gpu_amd import h5io
+    lib = h5io.lib()
+    names = declared_symbols("felbm_h5.h")
+    assert len(names) == 8
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(h5io.SIGNATURES) == names
 
 
 def test_version_and_error_strings():

@@ -1,0 +1,249 @@
+"""HDF5 files (felbm_gpu_amd/h5io.py over libfelbm_h5.so) in the reference
+driver's layout (src/felbm_gpu_main.cu:47-189).
+
+Pins:
+  - byte for byte (object modification times masked) against the reference's
+    three writers restated through the HDF5 C++ wrapper the reference calls
+    (oracle/h5_restated.cpp), for every combination of source type, gzip
+    level and on-disk type;
+  - read back by an independent reader (h5py, in the image's conda Python):
+    dataset names, dtypes, shapes, chunking, gzip level and values -- what
+    scripts/make_xdmf.py:63-232 relies on;
+  - the geometry `size` is the largest site coordinate + 1 (not the box);
+  - checkpoint round trip and the Vn-mismatch refusal (:176-181);
+  - particle files with and without the optional datasets;
+  - the driver loop's output cadence (:701-770).
+"""
+import json
+import os
+import subprocess
+import time
+
+import numpy as np
+import pytest
+
+from felbm_gpu_amd import h5io
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RESTATED = os.path.join(ROOT, "oracle", "h5_restated")
+CONDA_PY = "/opt/conda/bin/python3.9"
+
+needs_restated = pytest.mark.skipif(not os.path.exists(RESTATED), reason="oracle/h5_restated not built (no H5Cpp)")
+
+
+def _same_bytes(a, b, t0):
+    """Byte equality up to HDF5's object modification times: with the library's
+    default (H5Pset_obj_track_times on, as in the reference's files) every
+    dataset header carries its creation second as a little-endian uint32, which
+    two writers a second apart do not share.  Those words -- values between the
+    test's start and now -- are masked in both files before comparing."""
+    import time
+    t1 = int(time.time()) + 1
+    with open(a, "rb") as fa, open(b, "rb") as fb:
+        x, y = bytearray(fa.read()), bytearray(fb.read())
+    if len(x) != len(y):
+        return False
+    masked = 0
+    for buf in (x, y):
+        w = np.frombuffer(bytes(buf) + b"\0" * 3, dtype=np.uint8)
+        v = (w[:-3].astype(np.uint32) | (w[1:-2].astype(np.uint32) << 8) | (w[2:-1].astype(np.uint32) << 16)
+             | (w[3:].astype(np.uint32) << 24))
+        for j in np.nonzero((v >= t0 - 1) & (v <= t1))[0]:
+            buf[j:j + 4] = b"\0\0\0\0"
+            masked += 1
+    return x == y and masked > 0
+
+
+def _h5py_describe(path):
+    """name -> dtype, shape, chunks, compression, level, sha of the values, via h5py."""
+    if not os.path.exists(CONDA_PY):
+        pytest.skip("no h5py interpreter in this image")
+    code = r"""
+import h5py, hashlib, json, sys
+import numpy as np
+out = {}
+with h5py.File(sys.argv[1], "r") as f:
+    for k in f.keys():
+        d = f[k]
+        v = np.ascontiguousarray(d[()])
+        out[k] = {"dtype": d.dtype.str, "shape": list(d.shape), "chunks": list(d.chunks) if d.chunks else None,
+                  "compression": d.compression, "level": d.compression_opts,
+                  "sha": hashlib.sha256(v.tobytes()).hexdigest()}
+print(json.dumps(out))
+"""
+    r = subprocess.run([CONDA_PY, "-c", code, str(path)], capture_output=True, text=True, env={"PATH": "/usr/bin"})
+    if r.returncode != 0 and "No module named" in r.stderr:
+        pytest.skip("h5py is not importable")
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _fields(n, seed=1, f32=False):
+    rng = np.random.default_rng(seed)
+    F = rng.normal(size=(6, n))
+    F[1] = 0.5 * (1 + np.tanh(np.linspace(-4, 4, n)))       # smooth, compressible
+    return F.astype(np.float32) if f32 else F
+
+
+@needs_restated
+@pytest.mark.parametrize("src_f32", [0, 1])
+@pytest.mark.parametrize("deflate", [0, 1, 6])
+@pytest.mark.parametrize("out_f32", [0, 1])
+def test_fields_bytes_match_restated_writer(tmp_path, src_f32, deflate, out_f32):
+    t0 = int(time.time())
+    n = 3001
+    F = _fields(n, f32=bool(src_f32))
+    a, b, raw = tmp_path / "a.h5", tmp_path / "b.h5", tmp_path / "raw.bin"
+    F.tofile(raw)
+    h5io.write_fields(a, dict(zip(h5io.FIELD_KEYS, F)), deflate, bool(out_f32))
+    subprocess.run([RESTATED, "fields", str(b), str(n), str(raw), str(src_f32), str(deflate), str(out_f32)],
+                   check=True)
+    assert _same_bytes(a, b, t0)
+
+
+@needs_restated
+@pytest.mark.parametrize("deflate", [0, 4])
+def test_geometry_bytes_match_and_size_is_extent(tmp_path, deflate):
+    t0 = int(time.time())
+    solid = np.zeros((7, 6, 5), np.uint8)
+    solid[-1] = 1                     # the last z plane is solid: size_z = 6, not 7
+    solid[:, :, 4] = 1                # and the last x column: size_x = 4
+    solid[2, 3, 1] = 1
+    co = h5io.geometry_coords(solid)
+    assert co.shape == (int((solid == 0).sum()), 3) and co[1].tolist() == [1, 0, 0]
+    a, b, raw = tmp_path / "a.h5", tmp_path / "b.h5", tmp_path / "co.bin"
+    co.tofile(raw)
+    h5io.write_geometry(a, co, deflate)
+    subprocess.run([RESTATED, "geometry", str(b), str(len(co)), str(raw), str(deflate)], check=True)
+    assert _same_bytes(a, b, t0)
+    assert h5io.read_dataset(a, "size", np.int32).tolist() == [4, 6, 6]
+    assert np.array_equal(h5io.read_dataset(a, "coords", np.int32), co)
+
+
+@needs_restated
+def test_checkpoint_bytes_match_and_round_trip(tmp_path):
+    t0 = int(time.time())
+    rng = np.random.default_rng(3)
+    vn = 19 * 517
+    h, g = rng.random(vn), rng.random(vn)
+    a, b, raw = tmp_path / "a.h5", tmp_path / "b.h5", tmp_path / "hg.bin"
+    np.concatenate([h, g]).tofile(raw)
+    tm = 1234.5678901234567
+    h5io.write_checkpoint(a, h, g, 4242, tm)
+    subprocess.run([RESTATED, "checkpoint", str(b), str(vn), str(raw), "4242", repr(tm)], check=True)
+    assert _same_bytes(a, b, t0)
+    h2, g2, st, tm2 = h5io.read_checkpoint(a, vn)
+    assert np.array_equal(h2, h) and np.array_equal(g2, g) and st == 4242 and tm2 == tm
+    with pytest.raises(h5io.H5Error) as e:
+        h5io.read_checkpoint(a, vn + 19)
+    assert e.value.code == -3 and "mismatch" in str(e.value)
+
+
+@pytest.mark.parametrize("deflate,out_f32", [(0, False), (5, True)])
+def test_fields_layout_read_by_h5py(tmp_path, deflate, out_f32):
+    n = 2000
+    F = _fields(n, seed=2)
+    p = tmp_path / "output_100.h5"
+    h5io.write_fields(p, dict(zip(h5io.FIELD_KEYS, F)), deflate, out_f32)
+    d = _h5py_describe(p)
+    assert sorted(d) == sorted(h5io.FIELD_NAMES)
+    for i, name in enumerate(h5io.FIELD_NAMES):
+        e = d[name]
+        assert e["dtype"] == ("<f4" if out_f32 else "<f8") and e["shape"] == [n]
+        if deflate:
+            assert e["chunks"] == [n] and e["compression"] == "gzip" and e["level"] == deflate
+        else:
+            assert e["chunks"] is None and e["compression"] is None
+        assert e["sha"] == _sha(F[i].astype(np.float32) if out_f32 else F[i])
+
+
+def test_geometry_and_particles_read_by_h5py(tmp_path):
+    solid = (np.random.default_rng(4).random((5, 6, 7)) < 0.3).astype(np.uint8)
+    co = h5io.geometry_coords(solid)
+    gp = tmp_path / "geometry.h5"
+    h5io.write_geometry(gp, co, 3)
+    d = _h5py_describe(gp)
+    assert d["coords"]["dtype"] == "<i4" and d["coords"]["shape"] == [len(co), 3]
+    assert d["coords"]["chunks"] == [len(co), 3] and d["coords"]["level"] == 3
+    assert d["size"]["shape"] == [3] and d["coords"]["sha"] == _sha(co)
+    m = 50
+    rng = np.random.default_rng(5)
+    pos, vel = rng.random((m, 3)) * 6, rng.normal(size=(m, 3))
+    pp = tmp_path / "particles_10.h5"
+    h5io.write_particles(pp, pos, vel, dm=np.full(m, 1e-3), plane_id=np.arange(m) % 2)
+    d = _h5py_describe(pp)
+    assert list(d) == ["Dm", "id", "plane_id", "position", "velocity"]     # h5py lists names sorted
+    assert d["position"]["shape"] == [m, 3] and d["position"]["dtype"] == "<f8" and d["id"]["dtype"] == "<i4"
+    assert d["position"]["sha"] == _sha(pos) and d["velocity"]["sha"] == _sha(vel)
+
+
+def test_particles_round_trip_and_optional_datasets(tmp_path):
+    m = 40
+    rng = np.random.default_rng(6)
+    pos, vel = rng.random((m, 3)), rng.random((m, 3))
+    p = tmp_path / "p.h5"
+    h5io.write_particles(p, pos, vel)
+    f = h5io.read_particles(p)
+    assert sorted(f) == ["id", "position", "velocity"]
+    assert np.array_equal(f["position"], pos) and np.array_equal(f["id"], np.arange(m))
+    rh = rng.normal(size=(m, 3))
+    lr = rng.random(m)
+    h5io.write_particles(p, pos, vel, ids=np.arange(m)[::-1], dm=np.linspace(0, 1, m), rho_hat=rh, log_rho=lr)
+    f = h5io.read_particles(p)
+    assert np.array_equal(f["rho_hat"], rh) and np.array_equal(f["log_rho"], lr)
+    assert np.array_equal(f["id"], np.arange(m)[::-1]) and h5io.dataset_shape(p, "plane_id") is None
+    with pytest.raises(h5io.H5Error):
+        h5io.write_particles(p, pos, vel, rho_hat=rh)          # rho_hat without log_rho
+
+
+def test_errors_are_reported_not_printed(tmp_path, capfd):
+    with pytest.raises(h5io.H5Error) as e:
+        h5io.read_checkpoint(tmp_path / "missing.h5", 19)
+    assert e.value.code == -2
+    with pytest.raises(h5io.H5Error) as e:
+        h5io.write_fields(tmp_path / "x.h5", dict(zip(h5io.FIELD_KEYS, _fields(10))), deflate=11)
+    assert e.value.code == -1
+    assert "HDF5-DIAG" not in capfd.readouterr().err
+
+
+class _Recorder:
+    def __init__(self, tracers):
+        self.calls, self.tr = [], tracers
+
+    def has_tracers(self):
+        return self.tr
+
+    def write_checkpoint(self, path, t):
+        self.calls.append(("checkpoint", t, os.path.basename(path)))
+
+    def write_fields_h5(self, path, deflate, f32):
+        self.calls.append(("fields", deflate, f32, os.path.basename(path)))
+
+    def write_particles_h5(self, path):
+        self.calls.append(("particles", os.path.basename(path)))
+
+
+def test_output_schedule_cadence(tmp_path):
+    """felbm_gpu_main.cu:703-739: at t, checkpoint (t past the restart step and
+    a multiple of checkpoint_skip) with its particles file, then the fields
+    (file_skip), then the particles (particles_skip)."""
+    r = _Recorder(True)
+    s = h5io.OutputSchedule(tmp_path, "run", file_skip=4, checkpoint_skip=6, particles_skip=3, deflate=2, f32=True,
+                            restart_step=6)
+    for t in range(6, 13):
+        s.before_step(r, t)
+    assert r.calls == [("particles", "particles_6.h5"),
+                       ("fields", 2, True, "run_8.h5"),
+                       ("particles", "particles_9.h5"),
+                       ("checkpoint", 12, "checkpoint_12.h5"), ("particles", "particles_12.h5"),
+                       ("fields", 2, True, "run_12.h5")]
+    r2 = _Recorder(False)
+    s2 = h5io.OutputSchedule(tmp_path, file_skip=0, checkpoint_skip=5, particles_skip=1)
+    for t in range(0, 11):
+        s2.before_step(r2, t)
+    assert r2.calls == [("checkpoint", 5, "checkpoint_5.h5"), ("checkpoint", 10, "checkpoint_10.h5")]
