@@ -655,7 +655,7 @@ private:
   hipEvent_t ev_[10] = {};
   std::vector<Slot<real>> slots_;
   std::vector<int> slot_desc_;      // slot -> descriptor index
-  Inst<real> *d_desc_ = nullptr; int desc_cap_ = 0;
+  Inst<real> *d_desc_ = nullptr; int desc_cap_ = 0, ndesc_ = 0;
   int2 *d_map_ = nullptr;
   int2 *d_var_[8] = {};
   int nvar_[8] = {};
@@ -741,6 +741,7 @@ private:
       FM_HIP(hipMalloc((void **)&d_map_, sizeof(int2) * map_cap_));
     }
     if (!descs.empty()) FM_HIP(hipMemcpyAsync(d_desc_, descs.data(), sizeof(Inst<real>) * descs.size(), hipMemcpyHostToDevice, stream_));
+    ndesc_ = (int)descs.size();
     if (nmap_) FM_HIP(hipMemcpyAsync(d_map_, all.data(), sizeof(int2) * nmap_, hipMemcpyHostToDevice, stream_));
     {
       int2 *p = d_map_ + nmap_;
@@ -844,6 +845,10 @@ private:
       hipLaunchKernelGGL(k_open_commit<real>, dim3((n_ent_all_ + 255) / 256), blk, 0, stream_, d_desc_, d_ent_all_, n_ent_all_);
       FM_LAUNCH();
     }
+    if (ndesc_) {
+      hipLaunchKernelGGL(k_mass_lambda<real>, dim3(ndesc_), dim3(64), 0, stream_, d_desc_, 0, ndesc_, T_);
+      FM_LAUNCH();
+    }
     if (timed) FM_HIP(hipEventRecord(ev_[1], stream_));
     for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_key(k, 2, dim3(nvar_[k]), d_var_[k], stats); FM_LAUNCH(); }
     if (timed) FM_HIP(hipEventRecord(ev_[2], stream_));
@@ -922,6 +927,8 @@ private:
       hipLaunchKernelGGL(k_open_commit<real>, dim3((ne + 255) / 256), blk, 0, stream_, d_desc_, d_ent_all_ + S.e0, ne);
       FM_LAUNCH();
     }
+    hipLaunchKernelGGL(k_mass_lambda<real>, dim3(1), dim3(64), 0, stream_, d_desc_, di, 1, T_);
+    FM_LAUNCH();
     launch_key(variant(S), 2, grid, S.dmap, false, S.d.craw + export_off_ * ns, di); FM_LAUNCH();
     // the step's own moments pass must find this parity's partials at zero
     unsigned cur = cur_of(S);

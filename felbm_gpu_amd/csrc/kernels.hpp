@@ -119,7 +119,8 @@ struct ObEntry { int desc, site, ref, rkind, eid, pad; double rcoord; };
 // accumulator layout (unsigned long long, NSLOT slots per quantity)
 constexpr int ACC_MASS = 0;                 // [parity][5][NSLOT]: M.hi M.lo W.hi W.lo bad
 constexpr int ACC_STATS = 10*NSLOT;         // [12][NSLOT]: 5 sums x (hi,lo), max bits, bad
-constexpr int ACC_FLAGS = 22*NSLOT;         // [0] pending correction, [1] open-boundary step, [2] tracer step
+constexpr int ACC_FLAGS = 22*NSLOT;         // [0] pending correction, [1] open-boundary step, [2] tracer step,
+                                             // [3] lambda of the step (double bits), [4] lambda applies
 constexpr int ACC_MISC = 22*NSLOT + 8;      // [5][NSLOT] single-instance reductions
 constexpr int ACC_WORDS = 27*NSLOT + 8;
 
@@ -440,9 +441,34 @@ __device__ __forceinline__ double mass_lambda(const Inst<real> &d, unsigned cur,
 template <typename real>
 __device__ __forceinline__ real corrected(real c, bool apply, double lambda)
 {
-  double cm = (double)c;
-  double phi = cm*(1.0 - cm);
-  return (apply && phi > 0.0) ? (real)(cm + (-lambda*phi)) : c;
+  if constexpr (sizeof(real) == 4) {
+    // FP32 engine: the same correction in single precision
+    const float phi = c*(1.0f - c);
+    return (apply && phi > 0.0f) ? c - (float)lambda*phi : c;
+  } else {
+    double cm = (double)c;
+    double phi = cm*(1.0 - cm);
+    return (apply && phi > 0.0) ? (real)(cm + (-lambda*phi)) : c;
+  }
+}
+
+// lambda of the pending correction, once per instance and step (one wave per
+// descriptor in [first, first + count)), for the c stencil pass to read
+template <typename real>
+__global__ void __launch_bounds__(64) k_mass_lambda(const Inst<real> *__restrict__ D, int first, int count, unsigned T)
+{
+  const int q = first + (int)blockIdx.x;
+  if ((int)blockIdx.x >= count) return;
+  const Inst<real> &d = D[q];
+  if (!d.mass_on) return;
+  const unsigned cur = (T ^ (unsigned)d.par0) & 1u;
+  bool apply = false;
+  double lambda = 0.0;
+  if (ldg(d.acc, ACC_FLAGS)) lambda = mass_lambda(d, cur, apply);
+  if (threadIdx.x == 0) {
+    stg(d.acc, ACC_FLAGS + 3, (unsigned long long)__double_as_longlong(lambda));
+    stg(d.acc, ACC_FLAGS + 4, apply ? 1ull : 0ull);
+  }
 }
 
 template <typename real>
@@ -658,8 +684,12 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
   if (bm.x < 0) return;
   const Inst<real> &d = D[bm.x];
   const unsigned cur = (T ^ (unsigned)d.par0) & 1u;
+  (void)cur;
   bool apply = false; double lambda = 0.0;
-  if (d.mass_on && ldg(d.acc, ACC_FLAGS)) lambda = mass_lambda(d, cur, apply);
+  if (d.mass_on) {                 // set by k_mass_lambda for this step
+    apply = ldg(d.acc, ACC_FLAGS + 4) != 0ull;
+    lambda = __longlong_as_double((long long)ldg(d.acc, ACC_FLAGS + 3));
+  }
   const int i = bm.y + (int)threadIdx.x;
   if (i >= d.n) return;
   constexpr unsigned RB = sizeof(real);
@@ -675,14 +705,27 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
   for (int m = 1; m < Q; ++m) cn[m] = corrected<real>(bld<real>(rF, vo[m], F_CRAW*nsb), apply, lambda);
   real gx = 0, gy = 0, gz = 0, lapc = 0;
   const real bnd = d.bnd_coeff*c*((real)1 - c);
-  #pragma unroll
-  for (int m = 1; m < Q; ++m) {
-    const int o = OPP[m];
-    const bool hin = (fl >> m) & 1u, hip = (fl >> o) & 1u;
-    bool use; real dd = cd_num<real, HBB>(hin, hip, cn[m], cn[o], use);
-    real t = ((real)3.0*(real)0.5*W<real>(m))*dd;
-    acc_vec<real>(m, use ? t : (real)0, gx, gy, gz);
-    lapc += lap_term<real, HBB>(hin, hip, cn[m], cn[o], c, bnd, (real)3.0*W<real>(m));
+  if constexpr (sizeof(real) == 4) {
+    // FP32: a direction pair (m, opp m) contributes its central difference and
+    // its Laplacian term twice, so each pair is evaluated once
+    #pragma unroll
+    for (int m = 1; m < Q; m += 2) {
+      const int o = m + 1;
+      const bool hin = (fl >> m) & 1u, hip = (fl >> o) & 1u;
+      bool use; const float dd = cd_num<float, HBB>(hin, hip, cn[m], cn[o], use);
+      acc_vec<float>(m, use ? (3.0f*W<float>(m))*dd : 0.0f, gx, gy, gz);
+      lapc += lap_term<float, HBB>(hin, hip, cn[m], cn[o], c, bnd, 6.0f*W<float>(m));
+    }
+  } else {
+    #pragma unroll
+    for (int m = 1; m < Q; ++m) {
+      const int o = OPP[m];
+      const bool hin = (fl >> m) & 1u, hip = (fl >> o) & 1u;
+      bool use; real dd = cd_num<real, HBB>(hin, hip, cn[m], cn[o], use);
+      real t = ((real)3.0*(real)0.5*W<real>(m))*dd;
+      acc_vec<real>(m, use ? t : (real)0, gx, gy, gz);
+      lapc += lap_term<real, HBB>(hin, hip, cn[m], cn[o], c, bnd, (real)3.0*W<real>(m));
+    }
   }
   // open-boundary node: imposed rho (and u, p when the fields are imposed);
   // mu_bulk from the node's own raw c, as the reference's moments pass has it
@@ -694,7 +737,9 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
   const real cm = ovp ? ldg(ovp, 6) : c;
   real c_log = cm*((real)1 - cm);
   real rho = ovp ? ldg(ovp, 1) : d.rho1 + d.drho*c;
-  real inv = (real)1/((real)(1.0/3.0)*rho);
+  real irho, inv;                  // 1/rho and 1/(rho cs^2)
+  if constexpr (sizeof(real) == 4) { irho = __builtin_amdgcn_rcpf(rho); inv = 3.0f*irho; }
+  else { irho = (real)0; inv = (real)1/((real)(1.0/3.0)*rho); }
   real ux = bld<real>(rF, io, F_JX*nsb)*inv, uy = bld<real>(rF, io, F_JY*nsb)*inv, uz = bld<real>(rF, io, F_JZ*nsb)*inv;
   real mu = d.four_beta*c_log*((real)0.5 - cm);
   mu += -d.kappa*lapc;
@@ -703,10 +748,17 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
     // k_vel_press_corr skips nodes whose fields the boundary imposed
     ux = ldg(ovp, 3); uy = ldg(ovp, 4); uz = ldg(ovp, 5); p = ldg(ovp, 2);
   } else {
-    real mor = mu/rho;
-    ux = ux + (real)0.5*(d.gx + d.fx*d.ff/rho + mor*gx);
-    uy = uy + (real)0.5*(d.gy + d.fy*d.ff/rho + mor*gy);
-    uz = uz + (real)0.5*(d.gz + d.fz*d.ff/rho + mor*gz);
+    if constexpr (sizeof(real) == 4) {
+      const float mor = mu*irho, ffr = d.ff*irho;
+      ux = ux + 0.5f*(d.gx + d.fx*ffr + mor*gx);
+      uy = uy + 0.5f*(d.gy + d.fy*ffr + mor*gy);
+      uz = uz + 0.5f*(d.gz + d.fz*ffr + mor*gz);
+    } else {
+      real mor = mu/rho;
+      ux = ux + (real)0.5*(d.gx + d.fx*d.ff/rho + mor*gx);
+      uy = uy + (real)0.5*(d.gy + d.fy*d.ff/rho + mor*gy);
+      uz = uz + (real)0.5*(d.gz + d.fz*d.ff/rho + mor*gz);
+    }
     real u_dot_drho = d.drho*(ux*gx + uy*gy + uz*gz);
     p += (real)0.5*u_dot_drho*(real)(1.0/3.0);
   }
@@ -747,9 +799,16 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
   for (int m = 1; m < Q; ++m) mn[m] = bld<real>(rF, vo[m], F_MU*nsb);
   const real m0 = bld<real>(rF, io, F_MU*nsb);
   real s = 0;
-  #pragma unroll
-  for (int m = 1; m < Q; ++m)
-    s += lap_term<real, HBB>((fl >> m) & 1u, (fl >> OPP[m]) & 1u, mn[m], mn[OPP[m]], m0, (real)0, (real)3.0*W<real>(m));
+  if constexpr (sizeof(real) == 4) {
+    // FP32: both directions of a pair give the same term
+    #pragma unroll
+    for (int m = 1; m < Q; m += 2)
+      s += lap_term<float, HBB>((fl >> m) & 1u, (fl >> (m + 1)) & 1u, mn[m], mn[m + 1], m0, 0.0f, 6.0f*W<float>(m));
+  } else {
+    #pragma unroll
+    for (int m = 1; m < Q; ++m)
+      s += lap_term<real, HBB>((fl >> m) & 1u, (fl >> OPP[m]) & 1u, mn[m], mn[OPP[m]], m0, (real)0, (real)3.0*W<real>(m));
+  }
   bst_rec4(bld<real>(rF, io, F_C*nsb), bld<real>(rF, io, F_PS*nsb), s, m0, rF, io, F_REC*nsb);
 }
 
