@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
         ring2<float, OPEN>(d, rW, nb, io, vo2, fl2);
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
-          const Rec4<float> r1 = bld_rec4((const float *)nullptr, rF, vo[m], F_REC*nsb);
+          const Rec4<float> r1 = bld_rec3(rF, vo[m], F_REC*nsb);
           const Rec2<float> r2 = bld_rec2((const float *)nullptr, rF, vo2[m], F_REC*nsb);
           cN[m] = r1.c; pN[m] = r1.p; lN[m] = r1.l; c2[m] = r2.c; p2[m] = r2.p;
         }
@@ -97,11 +97,16 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       const float cp = fminf(fmaxf(crl, 0.0f), 1.0f);
       const float tau = frcp(cp*frcp(d.tau0) + (1.0f - cp)*frcp(d.tau1));
       const float relax = frcp(tau + 0.5f);
-      // pair numerators and gradients
-      float cdc[Q], cdp[Q], bdc[Q], bdp[Q];
+      const float c_over = 3.0f*c*irho;
+      const float a2 = 0.5f*(1.0f + c_over*mu), a4 = -0.5f*c_over, a5 = 0.5f*c_over;
+      const float hmob = 0.5f*d.mobility;
+      // per direction, once the stencils are in: the h-side terms
+      // Ph_k = a2 bd_c + a4 bd_p + mob/2 lap(mu)(x+e_k), and the g-side
+      // central (cd_c) and combined (cd_c + bd_c) differences of c
+      float Ph[Q], cdc[Q], dc2[Q];
       float e_dcx = 0, e_dcy = 0, e_dcz = 0, e_dpx = 0, e_dpy = 0, e_dpz = 0;
       float b_cx = 0, b_cy = 0, b_cz = 0, b_px = 0, b_py = 0, b_pz = 0;
-      cdc[0] = cdp[0] = bdc[0] = bdp[0] = 0.0f;
+      Ph[0] = hmob*lmu; cdc[0] = 0.0f; dc2[0] = 0.0f;
       #pragma unroll
       for (int m = 1; m < Q; m += 2) {
         const int o = m + 1;
@@ -113,13 +118,15 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
         acc_vec<float>(m, use ? c3w*nc : 0.0f, e_dcx, e_dcy, e_dcz);
         acc_vec<float>(m, use ? c3w*np : 0.0f, e_dpx, e_dpy, e_dpz);
         cdc[m] = use ? 0.5f*nc : 0.0f; cdc[o] = -cdc[m];
-        cdp[m] = use ? 0.5f*np : 0.0f; cdp[o] = -cdp[m];
-        bdc[m] = bd_num<float, HBB>(hin, h2, hip, hp2, cN[m], c2[m], cN[o], c2[o], c);
-        bdp[m] = bd_num<float, HBB>(hin, h2, hip, hp2, pN[m], p2[m], pN[o], p2[o], p);
-        bdc[o] = bd_num<float, HBB>(hip, hp2, hin, h2, cN[o], c2[o], cN[m], c2[m], c);
-        bdp[o] = bd_num<float, HBB>(hip, hp2, hin, h2, pN[o], p2[o], pN[m], p2[m], p);
-        acc_vec<float>(m, c3w*(bdc[m] - bdc[o]), b_cx, b_cy, b_cz);
-        acc_vec<float>(m, c3w*(bdp[m] - bdp[o]), b_px, b_py, b_pz);
+        const float bcm = bd_num<float, HBB>(hin, h2, hip, hp2, cN[m], c2[m], cN[o], c2[o], c);
+        const float bpm = bd_num<float, HBB>(hin, h2, hip, hp2, pN[m], p2[m], pN[o], p2[o], p);
+        const float bco = bd_num<float, HBB>(hip, hp2, hin, h2, cN[o], c2[o], cN[m], c2[m], c);
+        const float bpo = bd_num<float, HBB>(hip, hp2, hin, h2, pN[o], p2[o], pN[m], p2[m], p);
+        acc_vec<float>(m, c3w*(bcm - bco), b_cx, b_cy, b_cz);
+        acc_vec<float>(m, c3w*(bpm - bpo), b_px, b_py, b_pz);
+        Ph[m] = a2*bcm + a4*bpm + hmob*lN[m];
+        Ph[o] = a2*bco + a4*bpo + hmob*lN[o];
+        dc2[m] = cdc[m] + bcm; dc2[o] = cdc[o] + bco;
       }
       float gk[Q];
       float jx = 0, jy = 0, jz = 0;
@@ -145,7 +152,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
         vmax = (unsigned long long)__double_as_longlong(v2);
       }
       const float bfx = rho*d.gx + d.fx*d.ff, bfy = rho*d.gy + d.fy*d.ff, bfz = rho*d.gz + d.fz*d.ff;
-      const float drho_cs2 = drho*CS2, rho_cs2 = rho*CS2, c_over = 3.0f*c*irho;
+      const float drho_cs2 = drho*CS2, rho_cs2 = rho*CS2;
       const float e_fgx = mu*e_dcx + bfx, e_fgy = mu*e_dcy + bfy, e_fgz = mu*e_dcz + bfz;
       const float e_fhx = e_dcx - c_over*(e_dpx - e_fgx), e_fhy = e_dcy - c_over*(e_dpy - e_fgy), e_fhz = e_dcz - c_over*(e_dpz - e_fgz);
       const float f_fgx = mu*f_dcx + bfx, f_fgy = mu*f_dcy + bfy, f_fgz = mu*f_dcz + bfz;
@@ -158,13 +165,12 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       const float u_drm = drho_cs2*(ux*f_dcx + uy*f_dcy + uz*f_dcz);
       const float u_fg_f = ux*f_fgx + uy*f_fgy + uz*f_fgz;
       // h_k = G_k (K0 + a2 bd_c + a4 bd_p + a5 e_k.bf + mob avg_k)
-      const float K0 = c + 0.5f*u_term_h - u_fh_f;
-      const float a2 = 0.5f*(1.0f + c_over*mu), a4 = -0.5f*c_over, a5 = 0.5f*c_over;
+      //     = G_k (K0h + Ph_k + a5 e_k.bf),  K0h = K0 + mob/2 lap(mu)
+      const float K0h = c + 0.5f*u_term_h - u_fh_f + hmob*lmu;
       // eq_g = w p + wut (B0 - Bc cd_c) + G (u_force_g/2 +- (fp + mu cd_c)/2)
       // F_g  = wut (Bc (cd_c + bd_c) - u_drm) + G (mu/2 (cd_c + bd_c) + fp - u_fg_f)
       const float B0 = rho_cs2 + 0.5f*u_drho, Bc = 0.5f*drho_cs2, hmu = 0.5f*mu, hufg = 0.5f*u_force_g;
       const float gamma_u_sq = 1.5f*u_sq;
-      const float mob = d.mobility;
       auto storek = [&](float v, rsrc_t r, int k) {
         store_post<float, HBB, OPEN>(v, r, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
       };
@@ -181,15 +187,14 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
         const float u_term = uk*(3.0f + 4.5f*uk) - gamma_u_sq;
         const float wut = w*u_term;
         const float G = w + wut;
-        // 1/2 (lmu + lapmu(x+e_k)); a non-site neighbour fetched lmu itself
-        const float avgk = 0.5f*(lmu + (k ? lN[k] : lmu));
-        const float A = K0 + a2*bdc[k] + a4*bdp[k] + a5*fp + mob*avgk;
+        // Ph_k carries mob/2 lap(mu)(x+e_k); a non-site neighbour fetched lmu itself
+        const float A = K0h + Ph[k] + a5*fp;
         storek(G*A, rH, k);
-        const float cdk = cdc[k], dc2 = cdk + bdc[k];
+        const float cdk = cdc[k], dc2k = dc2[k];
         // BGK eq_g carries -(mu cd_c + fp)/2 (device_engine.cuh:516), MRT +(...)/2 (:598)
         const float eg = MRT ? w*p + wut*(B0 - Bc*cdk) + G*(0.5f*fp + hufg + hmu*cdk)
                              : w*p + wut*(B0 - Bc*cdk) + G*(hufg - 0.5f*fp - hmu*cdk);
-        const float fg = wut*(Bc*dc2 - u_drm) + G*(hmu*dc2 + fp - u_fg_f);
+        const float fg = wut*(Bc*dc2k - u_drm) + G*(hmu*dc2k + fp - u_fg_f);
         if constexpr (!MRT) {
           storek(gk[k] + relx*(eg - gk[k]) + fg, rGo, k);
         } else {

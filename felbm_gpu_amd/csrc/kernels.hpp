@@ -245,6 +245,13 @@ __device__ __forceinline__ Rec2<float> bld_rec2(const float *, rsrc_t r, unsigne
   const f32x2 f = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(vo*4u), (int)so, 0));
   return { f[0], f[1] };
 }
+// c, p*, lap(mu) of a record (its first 12 bytes)
+__device__ __forceinline__ Rec4<float> bld_rec3(rsrc_t r, unsigned vo, unsigned so)
+{
+  typedef float f32x3 __attribute__((ext_vector_type(3)));
+  const f32x3 f = __builtin_bit_cast(f32x3, __builtin_amdgcn_raw_buffer_load_b96(r, (int)(vo*4u), (int)so, 0));
+  return { f[0], f[1], f[2], 0.0f };
+}
 __device__ __forceinline__ Rec4<float> bld_rec4(const float *, rsrc_t r, unsigned vo, unsigned so)
 {
   const f32x4 f = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(vo*4u), (int)so, 0));
@@ -825,7 +832,8 @@ __device__ __forceinline__ void store_post(real v, rsrc_t r, int k, unsigned vo,
   real val = v;
   if constexpr (OPEN || !HBB) val = (fluid || (HBB && solid)) ? v : (real)0;
   (void)solid;
-  bst(val, r, io, (unsigned)k*nsb);
+  if constexpr (sizeof(real) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, val), r, (int)io, (int)((unsigned)k*nsb), 2);
+  else bst(val, r, io, (unsigned)k*nsb);
 }
 
 // ==== K4: fused collision, post-collision store (FP64, reference order) ==============
