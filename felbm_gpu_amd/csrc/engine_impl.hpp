@@ -116,6 +116,9 @@ public:
           words[(size_t)((((int64_t)z * g->ny) + y) * wpr + w)] = (start << 32) | mask;
         }
       }
+    // one zero word past the last row: a 16-byte row window read at the table's
+    // last word stays inside the buffer
+    words.push_back(0);
     if (cnt == 0) { set_error("geometry has no fluid sites"); return FELBM_EINVAL; }
     if (cnt >= (int64_t)1 << 31) { set_error("more than 2^31 sites"); return FELBM_EUNSUPPORTED; }
     S.n = (int)cnt; S.ns = (int)align_up((size_t)cnt, 64); S.wpr = wpr;

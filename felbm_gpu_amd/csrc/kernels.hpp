@@ -293,7 +293,10 @@ __device__ __forceinline__ void bst_rec4(double c, double p, double l, double m,
 // periodic axis).  Out-of-box coordinates are still wrapped, so every word
 // load stays inside the table.
 // yr and zr hold the wrapped y and z already scaled to row-word offsets.
-struct Nbhd { int xw[5], yr[5], zr[5]; unsigned ok; };
+// The rings read each row once, as a 16-byte window of the two words xb and
+// xb+1 that hold x-2..x+2; anyw says whether some lane of the wave is within
+// 2 of an x face, where x+dx may wrap outside that window.
+struct Nbhd { int xw[5], yr[5], zr[5]; unsigned ok, xb; bool anyw; };
 
 // offsets are within +-2, so one correction wraps any axis of length >= 2;
 // an axis of length 1 maps everything to 0
@@ -327,6 +330,18 @@ __device__ __forceinline__ void make_nbhd(const Inst<real> &d, int x, int y, int
     nb.zr[j] = (int)__umul24((unsigned)wrap_axis(z + j - 2, d.Nz, d.pz, c), plane);
     nb.ok |= (a ? 1u << j : 0u) | (b ? 1u << (5 + j) : 0u) | (c ? 1u << (10 + j) : 0u);
   }
+  nb.xb = (unsigned)(x >= 2 ? x - 2 : 0) >> 5;
+  nb.anyw = __ballot(x < 2 || x + 2 >= d.Nx) != 0ull;
+}
+
+// the words wb and wb+1 of the row starting at word `row` (a word past the
+// table reads as 0 and is never selected)
+struct RowWin { uint64_t a, b; };
+__device__ __forceinline__ RowWin row_win(rsrc_t rW, unsigned row, unsigned wb)
+{
+  typedef __attribute__((ext_vector_type(2))) unsigned long long u64x2;
+  const u64x2 v = __builtin_bit_cast(u64x2, __builtin_amdgcn_raw_buffer_load_b128(rW, (int)((row + wb)*8u), 0, 0));
+  return { v.x, v.y };
 }
 
 // The voxel at offset (dx,dy,dz), |d| <= 2.  Returns the byte offset (times
@@ -353,32 +368,77 @@ __device__ __forceinline__ unsigned nb_lookup(const Inst<real> &d, rsrc_t rW, co
   return fluid ? idx*(unsigned)sizeof(real) : self_off;
 }
 
+// nb_lookup with the voxel's row word taken from the row window L; `miss`
+// when the word is outside it (x+dx wrapped around a periodic x axis)
+template <typename real, bool OPEN>
+__device__ __forceinline__ unsigned nb_pick(const Nbhd &nb, const RowWin &L, int dx, int dy, int dz,
+                                            unsigned self_off, bool &fluid, bool &solid, bool &miss)
+{
+  const int xx = nb.xw[dx + 2];
+  const unsigned wi = (unsigned)xx >> 5;
+  miss = wi - nb.xb > 1u;
+  const uint64_t w = wi == nb.xb ? L.a : L.b;
+  const unsigned lo = (unsigned)w, b = (unsigned)xx & 31u;
+  const unsigned idx = (unsigned)(w >> 32) + (unsigned)__popc(lo & ((1u << b) - 1u));
+  bool f = (lo >> b) & 1u;
+  bool inbox = true;
+  if constexpr (OPEN) {
+    const unsigned bits = (1u << (dx + 2)) | (1u << (7 + dy)) | (1u << (12 + dz));
+    inbox = (nb.ok & bits) == bits;
+  }
+  fluid = f && inbox;
+  solid = !f && inbox;
+  return fluid ? idx*(unsigned)sizeof(real) : self_off;
+}
+
+// One ring (S = 1: x + e_m, S = 2: x + 2 e_m): each of its 9 rows is read once
+// as a window; the lanes whose x+dx wrapped (only in waves that have such a
+// lane) redo their lookup with the word read directly.
+template <typename real, bool OPEN, int S>
+__device__ __forceinline__ void ring_s(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
+                                       unsigned (&vo)[Q], unsigned &fl, unsigned &so)
+{
+  RowWin L[3][3];
+  #pragma unroll
+  for (int a = 0; a < 3; ++a)
+    #pragma unroll
+    for (int c = 0; c < 3; ++c)
+      L[a][c] = row_win(rW, (unsigned)nb.zr[2 + S*(c - 1)] + (unsigned)nb.yr[2 + S*(a - 1)], nb.xb);
+  vo[0] = self_off; fl = S == 1 ? 1u : 0u; so = 0u;
+  #pragma unroll
+  for (int m = 1; m < Q; ++m) {
+    bool f, s, miss;
+    vo[m] = nb_pick<real, OPEN>(nb, L[CY[m] + 1][CZ[m] + 1], S*CX[m], S*CY[m], S*CZ[m], self_off, f, s, miss);
+    if (CX[m] != 0 && nb.anyw && miss)
+      vo[m] = nb_lookup<real, OPEN>(d, rW, nb, S*CX[m], S*CY[m], S*CZ[m], self_off, f, s);
+    fl |= f ? (1u << m) : 0u; so |= s ? (1u << m) : 0u;
+  }
+}
+
+// The same rings with one row-word load per voxel: no window, and no second
+// round trip for the wrapped lanes.  The collision passes use this form: at 3
+// waves per SIMD they hide the extra round trip of ring_s worse than they
+// save on address instructions.
+template <typename real, bool OPEN, int S>
+__device__ __forceinline__ void ring_d(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
+                                       unsigned (&vo)[Q], unsigned &fl, unsigned &so)
+{
+  vo[0] = self_off; fl = S == 1 ? 1u : 0u; so = 0u;
+  #pragma unroll
+  for (int m = 1; m < Q; ++m) {
+    bool f, s;
+    vo[m] = nb_lookup<real, OPEN>(d, rW, nb, S*CX[m], S*CY[m], S*CZ[m], self_off, f, s);
+    fl |= f ? (1u << m) : 0u; so |= s ? (1u << m) : 0u;
+  }
+}
+
 // r = 1 ring: byte offsets vo[m] of x + e_m (own offset when not a site),
 // fluid bits fl and solid-in-box bits so (bit m)
 template <typename real, bool OPEN>
 __device__ __forceinline__ void ring1(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
                                       unsigned (&vo)[Q], unsigned &fl, unsigned &so)
 {
-  vo[0] = self_off; fl = 1u; so = 0u;
-  #pragma unroll
-  for (int m = 1; m < Q; ++m) {
-    bool f, s;
-    vo[m] = nb_lookup<real, OPEN>(d, rW, nb, CX[m], CY[m], CZ[m], self_off, f, s);
-    fl |= f ? (1u << m) : 0u; so |= s ? (1u << m) : 0u;
-  }
-}
-// r = 2 ring (x + 2 e_m): offsets and fluid bits
-template <typename real, bool OPEN>
-__device__ __forceinline__ void ring2(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
-                                      unsigned (&vo)[Q], unsigned &fl)
-{
-  vo[0] = self_off; fl = 0u;
-  #pragma unroll
-  for (int m = 1; m < Q; ++m) {
-    bool f, s;
-    vo[m] = nb_lookup<real, OPEN>(d, rW, nb, 2*CX[m], 2*CY[m], 2*CZ[m], self_off, f, s);
-    fl |= f ? (1u << m) : 0u;
-  }
+  ring_s<real, OPEN, 1>(d, rW, nb, self_off, vo, fl, so);
 }
 
 // ---- pull streaming ---------------------------------------------------------------
@@ -866,7 +926,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
     int x, y, z; decode(d, i, x, y, z);
     Nbhd nb; make_nbhd(d, x, y, z, nb);
     unsigned vo[Q], fl, so, fl2;
-    ring1<real, OPEN>(d, rW, nb, io, vo, fl, so);
+    ring_d<real, OPEN, 1>(d, rW, nb, io, vo, fl, so);
     // open-boundary node (ENT_* bits); a buffer node does not collide and
     // stores the equilibrium the reference writes into it (k_open_bnd_buf)
     int eb = -1;
@@ -895,7 +955,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
       real cN[Q], pN[Q], c2[Q], p2[Q], lN[Q];
       {
         unsigned vo2[Q];
-        ring2<real, OPEN>(d, rW, nb, io, vo2, fl2);
+        { unsigned so2; ring_d<real, OPEN, 2>(d, rW, nb, io, vo2, fl2, so2); }
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
           const Rec4<real> r1 = bld_rec4((const real *)nullptr, rF, vo[m], F_REC*nsb);
