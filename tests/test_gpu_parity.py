@@ -261,3 +261,22 @@ def test_no_bounce_back_all_fluid_bitexact():
     g.step(15)
     assert np.array_equal(o.download_state()[0], g.download_state()[0])
     assert np.array_equal(o.download_state()[1], g.download_state()[1])
+
+
+@pytest.mark.parametrize("nx,per_x", [(3, 1), (33, 1), (65, 1), (34, 0), (64, 0)])
+def test_row_word_windows_bitexact(nx, per_x):
+    """Row lengths around the 32-voxel words of the neighbour table: rows of
+    one, two and three words, x + dx crossing a word boundary and wrapping
+    around a periodic x axis (or leaving a closed one), against the oracle."""
+    rng = np.random.default_rng(nx)
+    solid = (rng.random((8, 7, nx)) < 0.3).astype(np.uint8)
+    solid[:, :, 0] = 0                       # fluid on both x faces
+    solid[:, :, -1] = 0
+    prm = W.derive_params(theta_deg=70.0, acceleration=(1e-5, 0, 0))
+    c0 = W.slab_c(solid, 0, 0, max(nx // 2, 1), 2.0)
+    _say("row windows nx", nx, "periodic x", per_x)
+    o, g = make_pair(solid, prm, 1, 1, periodic=(per_x, 1, 1), c0=c0)
+    o.run(6)
+    g.step(6)
+    assert np.array_equal(o.download_state()[0], g.download_state()[0])
+    assert np.array_equal(o.download_state()[1], g.download_state()[1])
