@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
     int x, y, z; decode(d, i, x, y, z);
     Nbhd nb; make_nbhd(d, x, y, z, nb);
     unsigned vo[Q], fl, so, fl2;
-    ring_d<float, OPEN, 1>(d, rW, nb, io, vo, fl, so);
+    ring_f<float, OPEN, 1>(d, rW, nb, io, vo, fl, so);
     // open-boundary node; a buffer node does not collide and stores the
     // equilibrium the reference writes into it (k_open_bnd_buf)
     int eb = -1;
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       float cN[Q], pN[Q], c2[Q], p2[Q], lN[Q];
       {
         unsigned vo2[Q];
-        { unsigned so2; ring_d<float, OPEN, 2>(d, rW, nb, io, vo2, fl2, so2); }
+        { unsigned so2; ring_f<float, OPEN, 2>(d, rW, nb, io, vo2, fl2, so2); }
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
           const Rec4<float> r1 = bld_rec3(rF, vo[m], F_REC*nsb);

@@ -368,67 +368,46 @@ __device__ __forceinline__ unsigned nb_lookup(const Inst<real> &d, rsrc_t rW, co
   return fluid ? idx*(unsigned)sizeof(real) : self_off;
 }
 
-// nb_lookup with the voxel's row word taken from the row window L; `miss`
-// when the word is outside it (x+dx wrapped around a periodic x axis)
-template <typename real, bool OPEN>
-__device__ __forceinline__ unsigned nb_pick(const Nbhd &nb, const RowWin &L, int dx, int dy, int dz,
-                                            unsigned self_off, bool &fluid, bool &solid, bool &miss)
-{
-  const int xx = nb.xw[dx + 2];
-  const unsigned wi = (unsigned)xx >> 5;
-  miss = wi - nb.xb > 1u;
-  const uint64_t w = wi == nb.xb ? L.a : L.b;
-  const unsigned lo = (unsigned)w, b = (unsigned)xx & 31u;
-  const unsigned idx = (unsigned)(w >> 32) + (unsigned)__popc(lo & ((1u << b) - 1u));
-  bool f = (lo >> b) & 1u;
-  bool inbox = true;
-  if constexpr (OPEN) {
-    const unsigned bits = (1u << (dx + 2)) | (1u << (7 + dy)) | (1u << (12 + dz));
-    inbox = (nb.ok & bits) == bits;
-  }
-  fluid = f && inbox;
-  solid = !f && inbox;
-  return fluid ? idx*(unsigned)sizeof(real) : self_off;
-}
-
-// One ring (S = 1: x + e_m, S = 2: x + 2 e_m): each of its 9 rows is read once
-// as a window; the lanes whose x+dx wrapped (only in waves that have such a
-// lane) redo their lookup with the word read directly.
+// One ring (S = 1: x + e_m, S = 2: x + 2 e_m).  Each of its 9 rows is read
+// once as a window; the wrapped window of each row that can need it (dy == 0
+// or dz == 0, entered off x) is read in the same round trip, in waves that
+// have a lane within 2 of an x face.  Reading the wrapped words only after
+// the local ones came back (a second round trip in half the waves) cost more
+// than it saved.
 template <typename real, bool OPEN, int S>
-__device__ __forceinline__ void ring_s(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
+__device__ __forceinline__ void ring_f(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
                                        unsigned (&vo)[Q], unsigned &fl, unsigned &so)
 {
-  RowWin L[3][3];
+  const int x2 = nb.xw[2];
+  const unsigned xf = x2 < 2 ? (unsigned)(d.Nx >= 2 ? d.Nx - 2 : 0) >> 5 : 0u;
+  RowWin L[3][3], F[3][3];
   #pragma unroll
   for (int a = 0; a < 3; ++a)
     #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      L[a][c] = row_win(rW, (unsigned)nb.zr[2 + S*(c - 1)] + (unsigned)nb.yr[2 + S*(a - 1)], nb.xb);
+    for (int c = 0; c < 3; ++c) {
+      const unsigned row = (unsigned)nb.zr[2 + S*(c - 1)] + (unsigned)nb.yr[2 + S*(a - 1)];
+      L[a][c] = row_win(rW, row, nb.xb);
+      F[a][c] = RowWin{0ull, 0ull};
+      if ((a == 1 || c == 1) && nb.anyw) F[a][c] = row_win(rW, row, xf);
+    }
   vo[0] = self_off; fl = S == 1 ? 1u : 0u; so = 0u;
   #pragma unroll
   for (int m = 1; m < Q; ++m) {
-    bool f, s, miss;
-    vo[m] = nb_pick<real, OPEN>(nb, L[CY[m] + 1][CZ[m] + 1], S*CX[m], S*CY[m], S*CZ[m], self_off, f, s, miss);
-    if (CX[m] != 0 && nb.anyw && miss)
-      vo[m] = nb_lookup<real, OPEN>(d, rW, nb, S*CX[m], S*CY[m], S*CZ[m], self_off, f, s);
-    fl |= f ? (1u << m) : 0u; so |= s ? (1u << m) : 0u;
-  }
-}
-
-// The same rings with one row-word load per voxel: no window, and no second
-// round trip for the wrapped lanes.  The collision passes use this form: at 3
-// waves per SIMD they hide the extra round trip of ring_s worse than they
-// save on address instructions.
-template <typename real, bool OPEN, int S>
-__device__ __forceinline__ void ring_d(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
-                                       unsigned (&vo)[Q], unsigned &fl, unsigned &so)
-{
-  vo[0] = self_off; fl = S == 1 ? 1u : 0u; so = 0u;
-  #pragma unroll
-  for (int m = 1; m < Q; ++m) {
-    bool f, s;
-    vo[m] = nb_lookup<real, OPEN>(d, rW, nb, S*CX[m], S*CY[m], S*CZ[m], self_off, f, s);
-    fl |= f ? (1u << m) : 0u; so |= s ? (1u << m) : 0u;
+    const int dx = S*CX[m], dy = S*CY[m], dz = S*CZ[m];
+    const RowWin &Lw = L[CY[m] + 1][CZ[m] + 1], &Fw = F[CY[m] + 1][CZ[m] + 1];
+    const int xx = nb.xw[dx + 2];
+    const unsigned wi = (unsigned)xx >> 5;
+    const uint64_t w = wi == nb.xb ? Lw.a : (wi == nb.xb + 1u ? Lw.b : (wi == xf ? Fw.a : Fw.b));
+    const unsigned lo = (unsigned)w, b = (unsigned)xx & 31u;
+    const unsigned idx = (unsigned)(w >> 32) + (unsigned)__popc(lo & ((1u << b) - 1u));
+    bool f = (lo >> b) & 1u, inbox = true;
+    if constexpr (OPEN) {
+      const unsigned bits = (1u << (dx + 2)) | (1u << (7 + dy)) | (1u << (12 + dz));
+      inbox = (nb.ok & bits) == bits;
+    }
+    const bool fluid = f && inbox, solid = !f && inbox;
+    vo[m] = fluid ? idx*(unsigned)sizeof(real) : self_off;
+    fl |= fluid ? (1u << m) : 0u; so |= solid ? (1u << m) : 0u;
   }
 }
 
@@ -438,7 +417,7 @@ template <typename real, bool OPEN>
 __device__ __forceinline__ void ring1(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
                                       unsigned (&vo)[Q], unsigned &fl, unsigned &so)
 {
-  ring_s<real, OPEN, 1>(d, rW, nb, self_off, vo, fl, so);
+  ring_f<real, OPEN, 1>(d, rW, nb, self_off, vo, fl, so);
 }
 
 // ---- pull streaming ---------------------------------------------------------------
@@ -742,8 +721,9 @@ __device__ __forceinline__ real lap_term(bool hin, bool hip, real fin, real fip,
 // k_pack_lap_c + k_lap_mf + k_mu_axpy + k_grad_cd_mf(c) + k_vel_press_corr
 // (device_engine.cuh:149-196).  exp != nullptr also writes rho, ux, uy, uz for
 // a field download (multiphase_gpu.cuh:704-757).
+// 7 waves per SIMD fit in 72 VGPRs without spilling (the compiler picks 74)
 template <typename real, bool HBB, bool OPEN>
-__global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_cfield(const Inst<real> *__restrict__ D,
                                                 const int2 *__restrict__ map, unsigned T,
                                                 real *exp, int exp_inst)
 {
@@ -926,7 +906,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
     int x, y, z; decode(d, i, x, y, z);
     Nbhd nb; make_nbhd(d, x, y, z, nb);
     unsigned vo[Q], fl, so, fl2;
-    ring_d<real, OPEN, 1>(d, rW, nb, io, vo, fl, so);
+    ring_f<real, OPEN, 1>(d, rW, nb, io, vo, fl, so);
     // open-boundary node (ENT_* bits); a buffer node does not collide and
     // stores the equilibrium the reference writes into it (k_open_bnd_buf)
     int eb = -1;
@@ -955,7 +935,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
       real cN[Q], pN[Q], c2[Q], p2[Q], lN[Q];
       {
         unsigned vo2[Q];
-        { unsigned so2; ring_d<real, OPEN, 2>(d, rW, nb, io, vo2, fl2, so2); }
+        { unsigned so2; ring_f<real, OPEN, 2>(d, rW, nb, io, vo2, fl2, so2); }
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
           const Rec4<real> r1 = bld_rec4((const real *)nullptr, rF, vo[m], F_REC*nsb);
