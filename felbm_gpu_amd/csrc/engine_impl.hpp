@@ -154,8 +154,9 @@ public:
     S.open = !(S.per[0] && S.per[1] && S.per[2]);
     d.h = (real *)(S.base + o_h); d.g0 = (real *)(S.base + o_g0); d.g1 = (real *)(S.base + o_g1);
     real *f = (real *)(S.base + o_f);
-    d.craw = f; d.p0 = f + ns; d.jx = f + 2 * ns; d.jy = f + 3 * ns; d.jz = f + 4 * ns;
-    d.c = f + 5 * ns; d.mu = f + 6 * ns; d.ps = f + 7 * ns; d.lapmu = f + 8 * ns;
+    d.craw = f; d.p0 = f + F_PJ * ns; d.jx = d.jy = d.jz = d.p0;      // [p0, jx, jy, jz] per site
+    d.c = d.ps = f + F_CP * ns;                                        // [c, p*] per site
+    d.mu = f + F_MU * ns; d.lapmu = f + F_LAPMU * ns;
     export_off_ = 9;   // exports at f + 9*ns .. 12*ns
     d.acc = (unsigned long long *)(S.base + o_a);
     set_params(S);
@@ -371,15 +372,14 @@ public:
     if (int e = field_pass(S)) return e;
     const size_t n = (size_t)S.n, ns = (size_t)S.ns;
     real *exp = S.d.craw + export_off_ * ns;
-    std::vector<real> t(4 * ns), cc(ns), pp(ns);
+    std::vector<real> t(4 * ns), cp(2 * ns);
     FM_HIP(hipMemcpyAsync(t.data(), exp, t.size() * sizeof(real), hipMemcpyDeviceToHost, stream_));
-    FM_HIP(hipMemcpyAsync(cc.data(), S.d.c, ns * sizeof(real), hipMemcpyDeviceToHost, stream_));
-    FM_HIP(hipMemcpyAsync(pp.data(), S.d.ps, ns * sizeof(real), hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipMemcpyAsync(cp.data(), S.d.c, cp.size() * sizeof(real), hipMemcpyDeviceToHost, stream_));   // [c, p*] pairs
     FM_HIP(hipStreamSynchronize(stream_));
     for (size_t i = 0; i < n; ++i) {
-      out[i] = (double)t[i]; out[n + i] = (double)cc[i];
+      out[i] = (double)t[i]; out[n + i] = (double)cp[2 * i];
       out[2 * n + i] = (double)t[ns + i]; out[3 * n + i] = (double)t[2 * ns + i]; out[4 * n + i] = (double)t[3 * ns + i];
-      out[5 * n + i] = (double)pp[i];
+      out[5 * n + i] = (double)cp[2 * i + 1];
     }
     return FELBM_OK;
   }

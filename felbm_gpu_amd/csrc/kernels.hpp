@@ -224,12 +224,14 @@ __device__ __forceinline__ void bst(float v, rsrc_t r, unsigned vo, unsigned so)
 __device__ __forceinline__ void bst(double v, rsrc_t r, unsigned vo, unsigned so)
 { __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, (int)vo, (int)so, 0); }
 
-// field block layout: [craw, p0, jx, jy, jz, c, mu, ps, lapmu] x ns, then
-// 4 x ns holding either the per-site records [c, p*, lapmu, mu] the mu-stencil
-// pass writes for the collision (one vector load per neighbour), or, between
-// steps, the exported rho, ux, uy, uz of a field download
-enum { F_CRAW = 0, F_P0 = 1, F_JX = 2, F_JY = 3, F_JZ = 4, F_C = 5, F_MU = 6, F_PS = 7, F_LAPMU = 8, F_REC = 9,
-       F_COUNT = 13 };
+// field block layout (ns values each): c_raw; [p0, jx, jy, jz] interleaved per
+// site (4 ns); [c, p*] interleaved per site (2 ns); mu; lapmu; then 4 x ns
+// holding either the per-site records [c, p*, lapmu, mu] the mu-stencil pass
+// writes for the collision (one vector load per neighbour), or, between steps,
+// the exported rho, ux, uy, uz of a field download.  A pass reads its own
+// site's inputs with one vector access per group: every vector memory
+// instruction costs the address unit the same whatever its width.
+enum { F_CRAW = 0, F_PJ = 1, F_CP = 5, F_MU = 7, F_LAPMU = 8, F_REC = 9, F_COUNT = 13 };
 
 // record loads/stores: (c, p*) or (c, p*, lapmu, mu) of one site, vo = site
 // byte offset in a scalar field (i * sizeof(real)); the record sits at 4*vo
@@ -238,6 +240,61 @@ enum { F_CRAW = 0, F_P0 = 1, F_JX = 2, F_JY = 3, F_JZ = 4, F_C = 5, F_MU = 6, F_
 // this toolchain, which narrows the load to a single dword.
 template <typename real> struct Rec2 { real c, p; };
 template <typename real> struct Rec4 { real c, p, l, m; };
+
+// [p0, jx, jy, jz] at byte offset 4*io and [c, p*] at 2*io of their blocks
+__device__ __forceinline__ void bst_pj(float p, float x, float y, float z, rsrc_t r, unsigned io, unsigned so)
+{
+  typedef __attribute__((ext_vector_type(4))) unsigned u4;
+  __builtin_amdgcn_raw_buffer_store_b128(u4{__builtin_bit_cast(unsigned, p), __builtin_bit_cast(unsigned, x),
+                                            __builtin_bit_cast(unsigned, y), __builtin_bit_cast(unsigned, z)},
+                                         r, (int)(io*4u), (int)so, 0);
+}
+__device__ __forceinline__ void bst_pj(double p, double x, double y, double z, rsrc_t r, unsigned io, unsigned so)
+{
+  typedef __attribute__((ext_vector_type(2))) unsigned u2;
+  typedef __attribute__((ext_vector_type(4))) unsigned u4;
+  const u2 a = __builtin_bit_cast(u2, p), b = __builtin_bit_cast(u2, x), c = __builtin_bit_cast(u2, y), e = __builtin_bit_cast(u2, z);
+  __builtin_amdgcn_raw_buffer_store_b128(u4{a[0], a[1], b[0], b[1]}, r, (int)(io*4u), (int)so, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(u4{c[0], c[1], e[0], e[1]}, r, (int)(io*4u + 16u), (int)so, 0);
+}
+__device__ __forceinline__ Rec4<float> bld_pj(const float *, rsrc_t r, unsigned io, unsigned so)
+{
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  const f32x4 f = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(io*4u), (int)so, 0));
+  return { f[0], f[1], f[2], f[3] };
+}
+__device__ __forceinline__ Rec4<double> bld_pj(const double *, rsrc_t r, unsigned io, unsigned so)
+{
+  typedef double f64x2 __attribute__((ext_vector_type(2)));
+  const f64x2 a = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(io*4u), (int)so, 0));
+  const f64x2 b = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(io*4u + 16u), (int)so, 0));
+  return { a[0], a[1], b[0], b[1] };
+}
+__device__ __forceinline__ void bst_cp(float c, float p, rsrc_t r, unsigned io, unsigned so)
+{
+  typedef __attribute__((ext_vector_type(2))) unsigned u2;
+  __builtin_amdgcn_raw_buffer_store_b64(u2{__builtin_bit_cast(unsigned, c), __builtin_bit_cast(unsigned, p)},
+                                        r, (int)(io*2u), (int)so, 0);
+}
+__device__ __forceinline__ void bst_cp(double c, double p, rsrc_t r, unsigned io, unsigned so)
+{
+  typedef __attribute__((ext_vector_type(2))) unsigned u2;
+  typedef __attribute__((ext_vector_type(4))) unsigned u4;
+  const u2 a = __builtin_bit_cast(u2, c), b = __builtin_bit_cast(u2, p);
+  __builtin_amdgcn_raw_buffer_store_b128(u4{a[0], a[1], b[0], b[1]}, r, (int)(io*2u), (int)so, 0);
+}
+__device__ __forceinline__ Rec2<float> bld_cp(const float *, rsrc_t r, unsigned io, unsigned so)
+{
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  const f32x2 f = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(io*2u), (int)so, 0));
+  return { f[0], f[1] };
+}
+__device__ __forceinline__ Rec2<double> bld_cp(const double *, rsrc_t r, unsigned io, unsigned so)
+{
+  typedef double f64x2 __attribute__((ext_vector_type(2)));
+  const f64x2 f = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(io*2u), (int)so, 0));
+  return { f[0], f[1] };
+}
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ Rec2<float> bld_rec2(const float *, rsrc_t r, unsigned vo, unsigned so)
@@ -560,8 +617,7 @@ __global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ 
       c += hv; p += gv;
       acc_vec<real>(k, gv, ux, uy, uz);
     }
-    bst(c, rF, io, F_CRAW*nsb); bst(p, rF, io, F_P0*nsb);
-    bst(ux, rF, io, F_JX*nsb); bst(uy, rF, io, F_JY*nsb); bst(uz, rF, io, F_JZ*nsb);
+    bst(c, rF, io, F_CRAW*nsb); bst_pj(p, ux, uy, uz, rF, io, F_PJ*nsb);
     if (d.mass_on) {              // k_mass_weight, device_engine.cuh:1109-1126
       double cm = (double)c;
       double phi = cm*(1.0 - cm); if (phi < 0.0) phi = 0.0;
@@ -625,8 +681,9 @@ __global__ void __launch_bounds__(256) k_open_fields(const Inst<real> *__restric
   (void)T;
   const long ns = d.ns;
   // the moments pass's sums at the mirrored vertex (k_moments, device_engine.cuh:116-135)
-  real c = ldg(d.craw, (long)e.ref), p = ldg(d.p0, (long)e.ref);
-  real ux = ldg(d.craw, 2*ns + e.ref), uy = ldg(d.craw, 3*ns + e.ref), uz = ldg(d.craw, 4*ns + e.ref);
+  real c = ldg(d.craw, (long)e.ref), p = ldg(d.craw, F_PJ*ns + 4*(long)e.ref);
+  real ux = ldg(d.craw, F_PJ*ns + 4*(long)e.ref + 1), uy = ldg(d.craw, F_PJ*ns + 4*(long)e.ref + 2);
+  real uz = ldg(d.craw, F_PJ*ns + 4*(long)e.ref + 3);
   const real cr = c;
   real rho = d.rho1 + d.drho*c;
   const real inv = (real)1/((real)(1.0/3.0)*rho);
@@ -787,10 +844,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
   real irho, inv;                  // 1/rho and 1/(rho cs^2)
   if constexpr (sizeof(real) == 4) { irho = __builtin_amdgcn_rcpf(rho); inv = 3.0f*irho; }
   else { irho = (real)0; inv = (real)1/((real)(1.0/3.0)*rho); }
-  real ux = bld<real>(rF, io, F_JX*nsb)*inv, uy = bld<real>(rF, io, F_JY*nsb)*inv, uz = bld<real>(rF, io, F_JZ*nsb)*inv;
+  const Rec4<real> pj = bld_pj((const real *)nullptr, rF, io, F_PJ*nsb);    // p0, jx, jy, jz
+  real ux = pj.p*inv, uy = pj.l*inv, uz = pj.m*inv;
   real mu = d.four_beta*c_log*((real)0.5 - cm);
   mu += -d.kappa*lapc;
-  real p = bld<real>(rF, io, F_P0*nsb);
+  real p = pj.c;
   if (eb >= 0 && (eb & ENT_BCPROC)) {
     // k_vel_press_corr skips nodes whose fields the boundary imposed
     ux = ldg(ovp, 3); uy = ldg(ovp, 4); uz = ldg(ovp, 5); p = ldg(ovp, 2);
@@ -809,7 +867,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
     real u_dot_drho = d.drho*(ux*gx + uy*gy + uz*gz);
     p += (real)0.5*u_dot_drho*(real)(1.0/3.0);
   }
-  bst(c, rF, io, F_C*nsb); bst(mu, rF, io, F_MU*nsb); bst(p, rF, io, F_PS*nsb);
+  bst_cp(c, p, rF, io, F_CP*nsb); bst(mu, rF, io, F_MU*nsb);
   if (exp && bm.x == exp_inst) {
     const long ns = d.ns;
     stg(exp, i, rho); stg(exp, ns + i, ux); stg(exp, 2*ns + i, uy); stg(exp, 3*ns + i, uz);
@@ -856,7 +914,8 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
     for (int m = 1; m < Q; ++m)
       s += lap_term<real, HBB>((fl >> m) & 1u, (fl >> OPP[m]) & 1u, mn[m], mn[OPP[m]], m0, (real)0, (real)3.0*W<real>(m));
   }
-  bst_rec4(bld<real>(rF, io, F_C*nsb), bld<real>(rF, io, F_PS*nsb), s, m0, rF, io, F_REC*nsb);
+  const Rec2<real> cp = bld_cp((const real *)nullptr, rF, io, F_CP*nsb);
+  bst_rec4(cp.c, cp.p, s, m0, rF, io, F_REC*nsb);
 }
 
 // Store one post-collision population k of the site at byte offset io into
