@@ -112,3 +112,24 @@ def test_stretching_matches_restatement_fp64():
     g.step(70)
     rows2, lost2 = g.tracers_stretch_rows()
     assert len(rows2) == 64 and lost2 == 6 and rows2[-1, 0] == 16 + 70
+
+
+def test_tracers_in_graph_replays_equal_eager():
+    """The tracers run on their own stream beside the mu pass and join before
+    the collision pass; captured into a graph and replayed, they move exactly
+    as in eager steps (positions, stretching state and rows)."""
+    runs = []
+    for graph in (False, True):
+        o, g, t, dm = setup(vskip=2)
+        g.tracers_stretching(True, phase_split=True, capacity=64)
+        if graph:
+            g.batch.step_graph(4, 3)
+        else:
+            g.step(12)
+        xyz, _, blk = g.tracers_download()
+        rh, lr = g.tracers_stretch_state()
+        rows, _ = g.tracers_stretch_rows()
+        runs.append((xyz, blk, rh, lr, rows))
+    _say("graph vs eager tracers: max|dx| %.3e" % float(np.abs(runs[0][0] - runs[1][0]).max()))
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
