@@ -131,7 +131,10 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       float gk[Q];
       float jx = 0, jy = 0, jz = 0;
       #pragma unroll
-      for (int k = 0; k < Q; ++k) { gk[k] = bld<float>(rGi, pull_off(k, vo, fl, io, nsb), pull_soff(k, nsb)); acc_vec<float>(k, gk[k], jx, jy, jz); }
+      // the streamed g, read with the non-temporal hint: its lines are not
+      // read again this step, and evicting them first keeps the neighbour
+      // records of the +-2 planes in L2 (0.826 -> 0.766 ms)
+      for (int k = 0; k < Q; ++k) { gk[k] = bld_nt<float>(rGi, pull_off(k, vo, fl, io, nsb), pull_soff(k, nsb)); acc_vec<float>(k, gk[k], jx, jy, jz); }
       const float f_dcx = 0.5f*(e_dcx + b_cx), f_dcy = 0.5f*(e_dcy + b_cy), f_dcz = 0.5f*(e_dcz + b_cz);
       const float f_dpx = 0.5f*(e_dpx + b_px), f_dpy = 0.5f*(e_dpy + b_py), f_dpz = 0.5f*(e_dpz + b_pz);
       const float inv = 3.0f*irho;
