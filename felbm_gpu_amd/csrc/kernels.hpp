@@ -242,57 +242,66 @@ template <typename real> struct Rec2 { real c, p; };
 template <typename real> struct Rec4 { real c, p, l, m; };
 
 // [p0, jx, jy, jz] at byte offset 4*io and [c, p*] at 2*io of their blocks
+// (AUX: the buffer instruction's cache policy, 2 = non-temporal)
+template <int AUX = 0>
 __device__ __forceinline__ void bst_pj(float p, float x, float y, float z, rsrc_t r, unsigned io, unsigned so)
 {
   typedef __attribute__((ext_vector_type(4))) unsigned u4;
   __builtin_amdgcn_raw_buffer_store_b128(u4{__builtin_bit_cast(unsigned, p), __builtin_bit_cast(unsigned, x),
                                             __builtin_bit_cast(unsigned, y), __builtin_bit_cast(unsigned, z)},
-                                         r, (int)(io*4u), (int)so, 0);
+                                         r, (int)(io*4u), (int)so, AUX);
 }
+template <int AUX = 0>
 __device__ __forceinline__ void bst_pj(double p, double x, double y, double z, rsrc_t r, unsigned io, unsigned so)
 {
   typedef __attribute__((ext_vector_type(2))) unsigned u2;
   typedef __attribute__((ext_vector_type(4))) unsigned u4;
   const u2 a = __builtin_bit_cast(u2, p), b = __builtin_bit_cast(u2, x), c = __builtin_bit_cast(u2, y), e = __builtin_bit_cast(u2, z);
-  __builtin_amdgcn_raw_buffer_store_b128(u4{a[0], a[1], b[0], b[1]}, r, (int)(io*4u), (int)so, 0);
-  __builtin_amdgcn_raw_buffer_store_b128(u4{c[0], c[1], e[0], e[1]}, r, (int)(io*4u + 16u), (int)so, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(u4{a[0], a[1], b[0], b[1]}, r, (int)(io*4u), (int)so, AUX);
+  __builtin_amdgcn_raw_buffer_store_b128(u4{c[0], c[1], e[0], e[1]}, r, (int)(io*4u + 16u), (int)so, AUX);
 }
+template <int AUX = 0>
 __device__ __forceinline__ Rec4<float> bld_pj(const float *, rsrc_t r, unsigned io, unsigned so)
 {
   typedef float f32x4 __attribute__((ext_vector_type(4)));
-  const f32x4 f = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(io*4u), (int)so, 0));
+  const f32x4 f = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(io*4u), (int)so, AUX));
   return { f[0], f[1], f[2], f[3] };
 }
+template <int AUX = 0>
 __device__ __forceinline__ Rec4<double> bld_pj(const double *, rsrc_t r, unsigned io, unsigned so)
 {
   typedef double f64x2 __attribute__((ext_vector_type(2)));
-  const f64x2 a = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(io*4u), (int)so, 0));
-  const f64x2 b = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(io*4u + 16u), (int)so, 0));
+  const f64x2 a = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(io*4u), (int)so, AUX));
+  const f64x2 b = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(io*4u + 16u), (int)so, AUX));
   return { a[0], a[1], b[0], b[1] };
 }
+template <int AUX = 0>
 __device__ __forceinline__ void bst_cp(float c, float p, rsrc_t r, unsigned io, unsigned so)
 {
   typedef __attribute__((ext_vector_type(2))) unsigned u2;
   __builtin_amdgcn_raw_buffer_store_b64(u2{__builtin_bit_cast(unsigned, c), __builtin_bit_cast(unsigned, p)},
-                                        r, (int)(io*2u), (int)so, 0);
+                                        r, (int)(io*2u), (int)so, AUX);
 }
+template <int AUX = 0>
 __device__ __forceinline__ void bst_cp(double c, double p, rsrc_t r, unsigned io, unsigned so)
 {
   typedef __attribute__((ext_vector_type(2))) unsigned u2;
   typedef __attribute__((ext_vector_type(4))) unsigned u4;
   const u2 a = __builtin_bit_cast(u2, c), b = __builtin_bit_cast(u2, p);
-  __builtin_amdgcn_raw_buffer_store_b128(u4{a[0], a[1], b[0], b[1]}, r, (int)(io*2u), (int)so, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(u4{a[0], a[1], b[0], b[1]}, r, (int)(io*2u), (int)so, AUX);
 }
+template <int AUX = 0>
 __device__ __forceinline__ Rec2<float> bld_cp(const float *, rsrc_t r, unsigned io, unsigned so)
 {
   typedef float f32x2 __attribute__((ext_vector_type(2)));
-  const f32x2 f = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(io*2u), (int)so, 0));
+  const f32x2 f = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(io*2u), (int)so, AUX));
   return { f[0], f[1] };
 }
+template <int AUX = 0>
 __device__ __forceinline__ Rec2<double> bld_cp(const double *, rsrc_t r, unsigned io, unsigned so)
 {
   typedef double f64x2 __attribute__((ext_vector_type(2)));
-  const f64x2 f = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(io*2u), (int)so, 0));
+  const f64x2 f = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(io*2u), (int)so, AUX));
   return { f[0], f[1] };
 }
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -328,13 +337,15 @@ __device__ __forceinline__ Rec4<double> bld_rec4(const double *, rsrc_t r, unsig
   return { __builtin_bit_cast(double, u2{a[0], a[1]}), __builtin_bit_cast(double, u2{a[2], a[3]}),
            __builtin_bit_cast(double, u2{b[0], b[1]}), __builtin_bit_cast(double, u2{b[2], b[3]}) };
 }
+template <int AUX = 0>
 __device__ __forceinline__ void bst_rec4(float c, float p, float l, float m, rsrc_t r, unsigned vo, unsigned so)
 {
   typedef __attribute__((ext_vector_type(4))) unsigned u4;
   u4 v = { __builtin_bit_cast(unsigned, c), __builtin_bit_cast(unsigned, p),
            __builtin_bit_cast(unsigned, l), __builtin_bit_cast(unsigned, m) };
-  __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(vo*4u), (int)so, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(vo*4u), (int)so, AUX);
 }
+template <int AUX = 0>
 __device__ __forceinline__ void bst_rec4(double c, double p, double l, double m, rsrc_t r, unsigned vo, unsigned so)
 {
   typedef __attribute__((ext_vector_type(2))) unsigned u2;
@@ -617,7 +628,8 @@ __global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ 
       c += hv; p += gv;
       acc_vec<real>(k, gv, ux, uy, uz);
     }
-    bst(c, rF, io, F_CRAW*nsb); bst_pj(p, ux, uy, uz, rF, io, F_PJ*nsb);
+    // [p0, j] is read once, by this site's own c pass: written non-temporal
+    bst(c, rF, io, F_CRAW*nsb); bst_pj<2>(p, ux, uy, uz, rF, io, F_PJ*nsb);
     if (d.mass_on) {              // k_mass_weight, device_engine.cuh:1109-1126
       double cm = (double)c;
       double phi = cm*(1.0 - cm); if (phi < 0.0) phi = 0.0;
@@ -844,7 +856,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
   real irho, inv;                  // 1/rho and 1/(rho cs^2)
   if constexpr (sizeof(real) == 4) { irho = __builtin_amdgcn_rcpf(rho); inv = 3.0f*irho; }
   else { irho = (real)0; inv = (real)1/((real)(1.0/3.0)*rho); }
-  const Rec4<real> pj = bld_pj((const real *)nullptr, rF, io, F_PJ*nsb);    // p0, jx, jy, jz
+  // p0, jx, jy, jz: read once, non-temporal (keeps L2 for the c_raw stencil)
+  const Rec4<real> pj = bld_pj<2>((const real *)nullptr, rF, io, F_PJ*nsb);
   real ux = pj.p*inv, uy = pj.l*inv, uz = pj.m*inv;
   real mu = d.four_beta*c_log*((real)0.5 - cm);
   mu += -d.kappa*lapc;
@@ -914,7 +927,9 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
     for (int m = 1; m < Q; ++m)
       s += lap_term<real, HBB>((fl >> m) & 1u, (fl >> OPP[m]) & 1u, mn[m], mn[OPP[m]], m0, (real)0, (real)3.0*W<real>(m));
   }
-  const Rec2<real> cp = bld_cp((const real *)nullptr, rF, io, F_CP*nsb);
+  // own [c, p*], read once: non-temporal, like the other single-use streams,
+  // so L2 keeps the stencil data (mu here, the records in the collision pass)
+  const Rec2<real> cp = bld_cp<2>((const real *)nullptr, rF, io, F_CP*nsb);
   bst_rec4(cp.c, cp.p, s, m0, rF, io, F_REC*nsb);
 }
 
