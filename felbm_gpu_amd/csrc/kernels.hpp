@@ -212,8 +212,9 @@ template <> __device__ __forceinline__ double bld<double>(rsrc_t r, unsigned vo,
 { return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0)); }
 template <> __device__ __forceinline__ uint64_t bld<uint64_t>(rsrc_t r, unsigned vo, unsigned so)
 { return __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0)); }
-// streaming loads read once per step: non-temporal, so they do not displace
-// the stencil fields the next passes re-read from L2
+// loads of data read once per step: non-temporal, so they do not displace
+// the stencil fields re-read from L2 (not for gathers whose lines adjacent
+// waves share through L2)
 template <typename T> __device__ __forceinline__ T bld_nt(rsrc_t r, unsigned vo, unsigned so);
 template <> __device__ __forceinline__ float bld_nt<float>(rsrc_t r, unsigned vo, unsigned so)
 { return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 2)); }
@@ -938,6 +939,8 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
 // x + e_k, or -- neighbour not a site -- from slot opp(k) of x as the
 // bounced-back population (halfway bounce-back off a solid) or zero (outside
 // the box, or no bounce-back).  vo is unused: the gather side resolves it.
+// The FP32 store carries the non-temporal hint: nothing reads the slot before
+// the next step, and the line need not displace the stencil data in L2.
 template <typename real, bool HBB, bool OPEN>
 __device__ __forceinline__ void store_post(real v, rsrc_t r, int k, unsigned vo, bool fluid, bool solid,
                                      unsigned io, unsigned nsb)
