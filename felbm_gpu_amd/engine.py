@@ -261,6 +261,8 @@ class MultiPhaseGPU:
                 self.set_forcing_factor(forcing_factor_at(timedep, period, t))
                 self.step()
                 self.apply_mass_correction()
+        if output is not None:
+            output.finish(self)
 
     # ---- HDF5 files (src/felbm_gpu_main.cu:47-189, :276, :395-410, :566-575, :706-770)
     def write_fields_h5(self, path, deflate=0, f32=False):

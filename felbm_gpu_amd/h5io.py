@@ -14,6 +14,7 @@ schedule of the reference's driver loop (src/felbm_gpu_main.cu:701-770):
 Loading fails loudly when the library has not been built.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -138,6 +139,24 @@ def write_particles(path, position, velocity, ids=None, dm=None, plane_id=None, 
                                           _d(lr)))
 
 
+def write_particles_csv(path, position, ids=None):
+    """particles_format = csv (felbm_gpu_main.cu:547-555): one plain-text line
+    id,x,y,z per tracer.  The reference's writer (ParticleManager) is not
+    available, so the header line and the digits are this engine's choice:
+    a '# id,x,y,z' line, then 17 significant digits (round-trip exact)."""
+    pos = np.asarray(position, dtype=np.float64).reshape(-1, 3)
+    ids = np.arange(len(pos)) if ids is None else np.asarray(ids).reshape(-1)
+    with open(path, "w") as f:
+        f.write("# id,x,y,z\n")
+        f.write("".join("%d,%.17g,%.17g,%.17g\n" % (int(i), p[0], p[1], p[2]) for i, p in zip(ids, pos)))
+
+
+def read_particles_csv(path):
+    """(ids, positions) of a file written by write_particles_csv."""
+    a = np.loadtxt(path, delimiter=",", comments="#", ndmin=2)
+    return a[:, 0].astype(np.int64), a[:, 1:4].copy()
+
+
 def dataset_shape(path, name):
     """shape tuple, or None when the file has no such dataset."""
     dims, nd = (_I64 * 2)(), _I()
@@ -171,20 +190,47 @@ def read_particles(path):
     return out
 
 
+def _g(x):
+    """A double as the driver's text streams print it (C++ default float format,
+    6 significant digits: the same as printf %g)."""
+    return "%g" % float(x)
+
+
 class OutputSchedule:
-    """The driver loop's file output (src/felbm_gpu_main.cu:701-770): before
+    """The driver loop's file output (src/felbm_gpu_main.cu:596-770): before
     step t, a checkpoint when checkpoint_skip divides t (t past the restart
     step), the fields when file_skip divides t, the particles when
-    particles_skip divides t.  Paths follow the reference's names under
-    output_dir."""
+    particles_skip divides t; when log_skip divides t, a row of the flow
+    monitor in timeseries_file (:620-626, :683-691), and when stretching_skip
+    divides t the stretching rows drained since the last time in
+    stretching_file (:635-671).  Paths follow the reference's names under
+    output_dir; the two text files are opened (truncated) with the schedule,
+    their column headers as the reference writes them."""
+
+    TS_HEADER = "# iter   max_speed   mean_ux   mean_uy   mean_uz   mean_v2   total_order_parameter\n"
+    ST_HEADER = ("# Lagrangian stretching of material line elements (Eq. 12), one row per step\n"
+                 "# step   lambda   mean_log_rho   var_log_rho   n_active   lambda_w   lambda_nw   n_w   n_nw\n")
 
     def __init__(self, output_dir, output_name="output", file_skip=0, checkpoint_skip=0, particles_skip=0,
-                 deflate=0, f32=False, restart_step=0):
+                 deflate=0, f32=False, restart_step=0, log_skip=0, timeseries_file=None, stretching_skip=0,
+                 stretching_file=None, particles_format="h5"):
         self.dir = os.fspath(output_dir)
         self.name = output_name
         self.file_skip, self.checkpoint_skip, self.particles_skip = int(file_skip), int(checkpoint_skip), int(
             particles_skip)
         self.deflate, self.f32, self.restart_step = int(deflate), bool(f32), int(restart_step)
+        # the periodic particle files; checkpoints always write HDF5 (they are restored from it)
+        self.pext = ".csv" if str(particles_format).lower() == "csv" else ".h5"
+        # log_skip defaults to file_skip, then 1; stretching_skip to log_skip (:620, :636)
+        self.log_skip = int(log_skip) or self.file_skip or 1
+        self.stretching_skip = int(stretching_skip) or self.log_skip
+        self.ts = self.st = None
+        if timeseries_file:
+            self.ts = open(os.path.join(self.dir, timeseries_file), "w")
+            self.ts.write(self.TS_HEADER)
+        if stretching_file:
+            self.st = open(os.path.join(self.dir, stretching_file), "w")
+            self.st.write(self.ST_HEADER)
         self.written = []
 
     def path(self, kind, t):
@@ -200,12 +246,48 @@ class OutputSchedule:
                 p = self.path("particles", t)
                 gpu.write_particles_h5(p)
                 self.written.append(p)
+        if self.ts is not None and t % self.log_skip == 0:
+            self.ts.write(self.stats_row(t, gpu.flow_stats(), gpu.n))
+            self.ts.flush()
         if self.file_skip and t % self.file_skip == 0:
             p = self.path("fields", t)
             gpu.write_fields_h5(p, self.deflate, self.f32)
             self.written.append(p)
         if self.particles_skip and gpu.has_tracers() and t % self.particles_skip == 0:
-            p = self.path("particles", t)
+            p = self.path("particles", t)[:-3] + self.pext
             if p not in self.written:
-                gpu.write_particles_h5(p)
+                if self.pext == ".csv":
+                    write_particles_csv(p, gpu.tracers_download()[0])
+                else:
+                    gpu.write_particles_h5(p)
                 self.written.append(p)
+        if self.st is not None and gpu.has_tracers() and t % self.stretching_skip == 0:
+            rows, _ = gpu.tracers_stretch_rows()
+            self.st.write("".join(self.stretch_row(r) for r in rows))
+            self.st.flush()
+
+    @staticmethod
+    def stats_row(t, st, n):
+        """iter, max |u|, <ux>, <uy>, <uz>, <u^2>, total c from flow_stats (:683-691)."""
+        inv = 1.0 / float(n)
+        return "%d  %s  %s  %s  %s  %s  %s\n" % (t, _g(math.sqrt(st[5])), _g(st[0] * inv), _g(st[1] * inv),
+                                                 _g(st[2] * inv), _g(st[3] * inv), _g(st[4]))
+
+    @staticmethod
+    def stretch_row(r):
+        """step, lambda, mean, var, n_active, lambda_w, lambda_nw, n_w, n_nw (:662-670)."""
+        return "%d  %s  %s  %s  %d  %s  %s  %d  %d\n" % (int(r[0]), _g(r[1]), _g(r[2]), _g(r[3]), int(r[4]), _g(r[5]),
+                                                       _g(r[6]), int(r[7]), int(r[8]))
+
+    def finish(self, gpu):
+        """End of a run (:781-783): drain the stretching rows not yet written."""
+        if self.st is not None and gpu.has_tracers():
+            rows, _ = gpu.tracers_stretch_rows()
+            self.st.write("".join(self.stretch_row(r) for r in rows))
+            self.st.flush()
+
+    def close(self):
+        for f in (self.ts, self.st):
+            if f is not None:
+                f.close()
+        self.ts = self.st = None

@@ -212,8 +212,18 @@ def test_errors_are_reported_not_printed(tmp_path, capfd):
 
 
 class _Recorder:
+    n = 4
+
     def __init__(self, tracers):
         self.calls, self.tr = [], tracers
+        self.step_rows = []
+
+    def flow_stats(self):
+        return np.array([0.4, -0.2, 0.0, 1e-7, 2.5, 9.0])
+
+    def tracers_stretch_rows(self):
+        rows, self.step_rows = np.array(self.step_rows).reshape(-1, 9), []
+        return rows, 0
 
     def has_tracers(self):
         return self.tr
@@ -247,3 +257,34 @@ def test_output_schedule_cadence(tmp_path):
     for t in range(0, 11):
         s2.before_step(r2, t)
     assert r2.calls == [("checkpoint", 5, "checkpoint_5.h5"), ("checkpoint", 10, "checkpoint_10.h5")]
+
+
+def test_timeseries_and_stretching_text_files(tmp_path):
+    """felbm_gpu_main.cu:620-691: one monitor row per log step (iter, max|u| =
+    sqrt of the max u^2, the means over n sites, total c) and the stretching
+    rows drained at their cadence and at the end, integers as integers."""
+    r = _Recorder(True)
+    s = h5io.OutputSchedule(tmp_path, log_skip=2, timeseries_file="series.txt", stretching_skip=3,
+                            stretching_file="stretching.txt")
+    for t in range(0, 5):
+        r.step_rows.append([t + 1, 1e-3, 0.25, 1e-4, 100, 2e-3, float("nan"), 40, 0])
+        s.before_step(r, t)
+    s.finish(r)
+    s.close()
+    ts = open(tmp_path / "series.txt").read().splitlines()
+    assert ts[0].startswith("# iter   max_speed") and len(ts) == 1 + 3          # t = 0, 2, 4
+    assert ts[1] == "0  3  0.1  -0.05  0  2.5e-08  2.5"
+    st = [l for l in open(tmp_path / "stretching.txt").read().splitlines() if not l.startswith("#")]
+    assert len(st) == 5 and st[0] == "1  0.001  0.25  0.0001  100  0.002  nan  40  0"
+    assert [int(l.split()[0]) for l in st] == [1, 2, 3, 4, 5]
+
+
+def test_particles_csv_round_trip(tmp_path):
+    """particles_format = csv: id,x,y,z lines, positions exact after reading."""
+    rng = np.random.default_rng(9)
+    pos = rng.random((30, 3)) * 100
+    p = tmp_path / "particles_7.csv"
+    h5io.write_particles_csv(p, pos)
+    ids, back = h5io.read_particles_csv(p)
+    assert np.array_equal(ids, np.arange(30)) and np.array_equal(back, pos)
+    assert open(p).readline().startswith("#")
