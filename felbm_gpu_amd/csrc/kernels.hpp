@@ -363,9 +363,9 @@ __device__ __forceinline__ void bst_rec4(double c, double p, double l, double m,
 // load stays inside the table.
 // yr and zr hold the wrapped y and z already scaled to row-word offsets.
 // The rings read each row once, as a 16-byte window of the two words xb and
-// xb+1 that hold x-2..x+2; anyw says whether some lane of the wave is within
-// 2 of an x face, where x+dx may wrap outside that window.
-struct Nbhd { int xw[5], yr[5], zr[5]; unsigned ok, xb; bool anyw; };
+// xb+1 that hold x-2..x+2; anyw[S-1] says whether some lane of the wave is
+// within S of an x face, where x+dx (|dx| <= S) may wrap outside that window.
+struct Nbhd { int xw[5], yr[5], zr[5]; unsigned ok, xb; bool anyw[2]; };
 
 // offsets are within +-2, so one correction wraps any axis of length >= 2;
 // an axis of length 1 maps everything to 0
@@ -400,7 +400,8 @@ __device__ __forceinline__ void make_nbhd(const Inst<real> &d, int x, int y, int
     nb.ok |= (a ? 1u << j : 0u) | (b ? 1u << (5 + j) : 0u) | (c ? 1u << (10 + j) : 0u);
   }
   nb.xb = (unsigned)(x >= 2 ? x - 2 : 0) >> 5;
-  nb.anyw = __ballot(x < 2 || x + 2 >= d.Nx) != 0ull;
+  nb.anyw[0] = __ballot(x < 1 || x + 1 >= d.Nx) != 0ull;
+  nb.anyw[1] = __ballot(x < 2 || x + 2 >= d.Nx) != 0ull;
 }
 
 // the words wb and wb+1 of the row starting at word `row` (a word past the
@@ -457,7 +458,7 @@ __device__ __forceinline__ void ring_f(const Inst<real> &d, rsrc_t rW, const Nbh
       const unsigned row = (unsigned)nb.zr[2 + S*(c - 1)] + (unsigned)nb.yr[2 + S*(a - 1)];
       L[a][c] = row_win(rW, row, nb.xb);
       F[a][c] = RowWin{0ull, 0ull};
-      if ((a == 1 || c == 1) && nb.anyw) F[a][c] = row_win(rW, row, xf);
+      if ((a == 1 || c == 1) && nb.anyw[S - 1]) F[a][c] = row_win(rW, row, xf);
     }
   vo[0] = self_off; fl = S == 1 ? 1u : 0u; so = 0u;
   #pragma unroll
