@@ -75,7 +75,7 @@ def main():
                     help="only run the device-to-device copy probe of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU figures are not part of this build "
+        print("bench.py: this engine runs on one GPU; runs across several GPUs are not supported "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

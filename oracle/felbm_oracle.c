@@ -1,7 +1,7 @@
 /*
  * felbm_oracle.c -- FP64 CPU restatement of the felbm Lee-Liu D3Q19 step.
  *
- * TEST INFRASTRUCTURE: the parity checker and CPU baseline for the HIP
+ * TEST INFRASTRUCTURE: the parity oracle and CPU baseline for the HIP
  * engine; never linked into the product.  Built with -ffp-contract=off so
  * every operation rounds exactly as written (the HIP FP64 build uses the same
  * flag and the same expression order, which makes the two bit-identical).
@@ -233,7 +233,7 @@ static void build_tables(orc_engine *E)
     }
     /* streaming source (the StreamingOperator rows reduced as in
      * multiphase_gpu.cuh:200-208).  The operator lives in the absent tree;
-     * this build defines it: rest = self; bulk pull from x - e_k; halfway
+     * this engine defines it: rest = self; bulk pull from x - e_k; halfway
      * bounce-back from slot opp(k) of the same site when x - e_k is a solid
      * voxel inside the box; empty (-2) when x - e_k is outside a
      * non-periodic box face. */

@@ -1,7 +1,7 @@
 /*
  * felbm_oracle.h -- FP64 CPU restatement of the felbm Lee-Liu D3Q19 time step.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity oracle for the HIP
  * engine and the CPU baseline timed beside it.  Only tests/, the smoke() entry
  * point and bench.py's cpu_baseline leg may load it; the product path never
  * does.
@@ -32,7 +32,7 @@ extern "C" {
 
 #define ORC_Q 19
 
-/* ---- lattice: the SubDomain semantics this build defines ----------------
+/* ---- lattice: the SubDomain semantics this engine defines ---------------
  * Sites are the fluid voxels of an Nx*Ny*Nz box, numbered in x-fastest
  * lexicographic order (voxel v = x + Nx*(y + Ny*z)).  Solid voxels are not
  * sites; coords_to_idx of a solid voxel is the sentinel -1 and is_solid(-1)

@@ -1,4 +1,4 @@
-// h5_restated.cpp -- TEST INFRASTRUCTURE ONLY (the checker of
+// h5_restated.cpp -- TEST INFRASTRUCTURE ONLY (the oracle for
 // felbm_gpu_amd/csrc/h5io.c; the product never runs it).
 //
 // Restates the reference driver's HDF5 writers through the HDF5 C++ wrapper

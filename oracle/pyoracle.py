@@ -1,7 +1,7 @@
 """ctypes wrapper of the FP64 CPU oracle (oracle/liborc.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker.  The product never loads it.
+bench.py's cpu_baseline leg, as the oracle.  The product never loads it.
 """
 import ctypes as C
 import os
@@ -116,7 +116,7 @@ def to_orc_params(p):
 
 
 class Oracle:
-    """FP64 CPU restatement of MultiPhaseGPU (the checker)."""
+    """FP64 CPU restatement of MultiPhaseGPU (the oracle)."""
 
     def __init__(self, solid, params, use_mrt=False, fuse_collision=True, stream_inplace=False, mass_mode=0,
                  periodic=(1, 1, 1), use_halfway_bb=True, open_bnd=None):

@@ -1,7 +1,7 @@
-"""NumPy restatement of the tracer update (the checker for the engine's
+"""NumPy restatement of the tracer update (the oracle for the engine's
 tracer kernel).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker; the product
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the oracle; the product
 never loads it.
 
 The reference advects its tracers with felbm_local's ParticleManager, which is
