@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=1000,
                     help="steps of the 48-instance batched sweep (0 skips it)")
     ap.add_argument("--serve", type=int, default=16, help="requests of the served stream (0 skips it)")
+    ap.add_argument("--err-steps", type=int, default=1000,
+                    help="steps of the FP32-vs-FP64 field comparison on the same lattice (0 skips it)")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="only run the device-to-device copy probe of this size (counter calibration)")
     args = ap.parse_args()
@@ -147,6 +149,26 @@ def main():
     def log(msg):
         print("[bench] " + msg, file=sys.stderr, flush=True)
 
+    # FP32 error at full size: the FP32 engine against the FP64 engine (bit-identical
+    # to the FP64 oracle wherever the two are compared) from the same initial state,
+    # max |difference| of each downloaded field after err_steps steps
+    f32_err = None
+    if args.err_steps > 0 and w["precision"] == "f32":
+        eng.free()
+        fields = {}
+        for prec in ("f64", "f32"):
+            e2 = MultiPhaseGPU(precision=prec).init(w["solid"], w["params"], w["switches"])
+            e2.init_rest(w["c0"], w["p0"])
+            e2.record_target_mass()
+            e2.step(args.err_steps)
+            fields[prec] = e2.download()
+            e2.free()
+        f32_err = {"steps": args.err_steps}
+        for k in ("rho", "c", "ux", "uy", "uz", "p"):
+            f32_err[k] = float(np.abs(fields["f32"][k] - fields["f64"][k]).max())
+        del fields
+        log("fp32 vs fp64 after %d steps: %s" % (args.err_steps, json.dumps(f32_err)))
+
     sweep = None
     if args.sweep_steps > 0:
         eng.free()
@@ -180,6 +202,7 @@ def main():
         "kernels_ms": {k: v["ms"] for k, v in kern.items()},
         "mass_drift": mass_drift,
         "with_tracers_MLUPS": tr_mlups,
+        "fp32_max_abs_diff_vs_fp64": f32_err,
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched_sweep": sweep,
