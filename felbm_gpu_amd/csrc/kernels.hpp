@@ -536,6 +536,13 @@ __device__ __forceinline__ void neighbours2(const Inst<real> &d, const Nbhd &nb,
 }
 
 template <typename real> __device__ __forceinline__ real W(int k) { return (real)WD[k]; }
+// FP32 weights: 1/3, 1/18, 1/36 rounded to nearest sum to 1 + 2^-26, and the
+// equilibria built on them then add that fraction of c (and of p) at every site
+// every step -- a steady mass gain that the mass correction moves onto the
+// interface.  These sum to exactly 1: 1/18 one ulp below its nearest float,
+// 1/36 its half, and the rest weight the exact remainder 1 - 12 w1.
+constexpr float WF0 = 0x1.555558p-2f, WF1 = 0x1.c71c70p-5f, WF2 = 0x1.c71c70p-6f;
+template <> __device__ __forceinline__ float W<float>(int k) { return k == 0 ? WF0 : (k < 7 ? WF1 : WF2); }
 
 // ---- mass-correction scalar from the partial sums of this step ------------------
 // lambda = (M - M0)/W applied when W > 0 and M != M0 (multiphase_gpu.cuh:689-700)
@@ -739,7 +746,7 @@ template <typename real>
 __device__ __forceinline__ void plain_eq(int k, real ux, real uy, real uz, real pp, real rho, real c,
                                          real &h, real &g)
 {
-  const real w = (real)WD[k];
+  const real w = W<real>(k);
   const real u_sq = ux*ux + uy*uy + uz*uz;
   const real uk = ux*(real)CX[k] + uy*(real)CY[k] + uz*(real)CZ[k];
   const real u_term = uk*((real)3.0 + (real)4.5*uk) - (real)1.5*u_sq;
