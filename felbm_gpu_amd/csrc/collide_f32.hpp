@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
     int x, y, z; decode(d, i, x, y, z);
     Nbhd nb; make_nbhd(d, x, y, z, nb);
     unsigned vo[Q], fl, so, fl2;
-    ring_f<float, OPEN, 1>(d, rW, nb, io, vo, fl, so);
+    ring_f<float, OPEN, 1, false>(d, rW, nb, io, vo, fl, so);
     // open-boundary node; a buffer node does not collide and stores the
     // equilibrium the reference writes into it (k_open_bnd_buf)
     int eb = -1;
@@ -77,17 +77,20 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       }
     } else {
       float cN[Q], pN[Q], c2[Q], p2[Q], lN[Q];
+      const Rec4<float> r0 = bld_rec4((const float *)nullptr, rF, io, F_REC*nsb);
       {
+        // a neighbour that is not a site takes the own record's values
         unsigned vo2[Q];
-        { unsigned so2; ring_f<float, OPEN, 2>(d, rW, nb, io, vo2, fl2, so2); }
+        { unsigned so2; ring_f<float, OPEN, 2, false>(d, rW, nb, io, vo2, fl2, so2); }
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
           const Rec4<float> r1 = bld_rec3(rF, vo[m], F_REC*nsb);
           const Rec2<float> r2 = bld_rec2((const float *)nullptr, rF, vo2[m], F_REC*nsb);
-          cN[m] = r1.c; pN[m] = r1.p; lN[m] = r1.l; c2[m] = r2.c; p2[m] = r2.p;
+          const bool s1 = (fl >> m) & 1u, s2 = (fl2 >> m) & 1u;
+          cN[m] = s1 ? r1.c : r0.c; pN[m] = s1 ? r1.p : r0.p; lN[m] = s1 ? r1.l : r0.l;
+          c2[m] = s2 ? r2.c : r0.c; p2[m] = s2 ? r2.p : r0.p;
         }
       }
-      const Rec4<float> r0 = bld_rec4((const float *)nullptr, rF, io, F_REC*nsb);
       const float c = r0.c, p = r0.p, mu = r0.m, lmu = r0.l;
       const float drho = d.drho;
       const float CS2 = 1.0f/3.0f;

@@ -444,7 +444,12 @@ __device__ __forceinline__ unsigned nb_lookup(const Inst<real> &d, rsrc_t rW, co
 // have a lane within 2 of an x face.  Reading the wrapped words only after
 // the local ones came back (a second round trip in half the waves) cost more
 // than it saved.
-template <typename real, bool OPEN, int S>
+// SELF = false: a neighbour that is not a site gets the offset of the next
+// site along its row instead of the own offset, so a wave's load for one
+// direction stays within the lines of that row (the caller substitutes the
+// own value); the own offset would add the lines of the own row to every
+// such load.
+template <typename real, bool OPEN, int S, bool SELF = true>
 __device__ __forceinline__ void ring_f(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
                                        unsigned (&vo)[Q], unsigned &fl, unsigned &so)
 {
@@ -476,7 +481,7 @@ __device__ __forceinline__ void ring_f(const Inst<real> &d, rsrc_t rW, const Nbh
       inbox = (nb.ok & bits) == bits;
     }
     const bool fluid = f && inbox, solid = !f && inbox;
-    vo[m] = fluid ? idx*(unsigned)sizeof(real) : self_off;
+    vo[m] = (fluid || !SELF) ? idx*(unsigned)sizeof(real) : self_off;
     fl |= fluid ? (1u << m) : 0u; so |= solid ? (1u << m) : 0u;
   }
 }
@@ -799,9 +804,8 @@ __device__ __forceinline__ real lap_term(bool hin, bool hip, real fin, real fip,
 // k_pack_lap_c + k_lap_mf + k_mu_axpy + k_grad_cd_mf(c) + k_vel_press_corr
 // (device_engine.cuh:149-196).  exp != nullptr also writes rho, ux, uy, uz for
 // a field download (multiphase_gpu.cuh:704-757).
-// 7 waves per SIMD fit in 72 VGPRs without spilling (the compiler picks 74)
 template <typename real, bool HBB, bool OPEN>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_cfield(const Inst<real> *__restrict__ D,
+__global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D,
                                                 const int2 *__restrict__ map, unsigned T,
                                                 real *exp, int exp_inst)
 {
@@ -823,11 +827,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
   int x, y, z; decode(d, i, x, y, z);
   Nbhd nb; make_nbhd(d, x, y, z, nb);
   unsigned vo[Q], fl, so;
-  ring1<real, OPEN>(d, rW, nb, io, vo, fl, so);
-  const real c = corrected<real>(bld<real>(rF, io, F_CRAW*nsb), apply, lambda);
+  ring_f<real, OPEN, 1, false>(d, rW, nb, io, vo, fl, so);
+  const real craw = bld<real>(rF, io, F_CRAW*nsb);
+  const real c = corrected<real>(craw, apply, lambda);
   real cn[Q];
   #pragma unroll
-  for (int m = 1; m < Q; ++m) cn[m] = corrected<real>(bld<real>(rF, vo[m], F_CRAW*nsb), apply, lambda);
+  for (int m = 1; m < Q; ++m) {
+    const real v = bld<real>(rF, vo[m], F_CRAW*nsb);
+    cn[m] = corrected<real>(((fl >> m) & 1u) ? v : craw, apply, lambda);
+  }
   real gx = 0, gy = 0, gz = 0, lapc = 0;
   const real bnd = d.bnd_coeff*c*((real)1 - c);
   if constexpr (sizeof(real) == 4) {
@@ -920,11 +928,14 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
   int x, y, z; decode(d, i, x, y, z);
   Nbhd nb; make_nbhd(d, x, y, z, nb);
   unsigned vo[Q], fl, so;
-  ring1<real, OPEN>(d, rW, nb, io, vo, fl, so);
+  ring_f<real, OPEN, 1, false>(d, rW, nb, io, vo, fl, so);
+  const real m0 = bld<real>(rF, io, F_MU*nsb);
   real mn[Q];
   #pragma unroll
-  for (int m = 1; m < Q; ++m) mn[m] = bld<real>(rF, vo[m], F_MU*nsb);
-  const real m0 = bld<real>(rF, io, F_MU*nsb);
+  for (int m = 1; m < Q; ++m) {
+    const real v = bld<real>(rF, vo[m], F_MU*nsb);
+    mn[m] = ((fl >> m) & 1u) ? v : m0;
+  }
   real s = 0;
   if constexpr (sizeof(real) == 4) {
     // FP32: both directions of a pair give the same term
@@ -991,7 +1002,7 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
     int x, y, z; decode(d, i, x, y, z);
     Nbhd nb; make_nbhd(d, x, y, z, nb);
     unsigned vo[Q], fl, so, fl2;
-    ring_f<real, OPEN, 1>(d, rW, nb, io, vo, fl, so);
+    ring_f<real, OPEN, 1, false>(d, rW, nb, io, vo, fl, so);
     // open-boundary node (ENT_* bits); a buffer node does not collide and
     // stores the equilibrium the reference writes into it (k_open_bnd_buf)
     int eb = -1;
@@ -1018,17 +1029,20 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
       }
     } else {
       real cN[Q], pN[Q], c2[Q], p2[Q], lN[Q];
+      const Rec4<real> r0 = bld_rec4((const real *)nullptr, rF, io, F_REC*nsb);
       {
+        // a neighbour that is not a site takes the own record's values
         unsigned vo2[Q];
-        { unsigned so2; ring_f<real, OPEN, 2>(d, rW, nb, io, vo2, fl2, so2); }
+        { unsigned so2; ring_f<real, OPEN, 2, false>(d, rW, nb, io, vo2, fl2, so2); }
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
           const Rec4<real> r1 = bld_rec4((const real *)nullptr, rF, vo[m], F_REC*nsb);
           const Rec2<real> r2 = bld_rec2((const real *)nullptr, rF, vo2[m], F_REC*nsb);
-          cN[m] = r1.c; pN[m] = r1.p; lN[m] = r1.l; c2[m] = r2.c; p2[m] = r2.p;
+          const bool s1 = (fl >> m) & 1u, s2 = (fl2 >> m) & 1u;
+          cN[m] = s1 ? r1.c : r0.c; pN[m] = s1 ? r1.p : r0.p; lN[m] = s1 ? r1.l : r0.l;
+          c2[m] = s2 ? r2.c : r0.c; p2[m] = s2 ? r2.p : r0.p;
         }
       }
-      const Rec4<real> r0 = bld_rec4((const real *)nullptr, rF, io, F_REC*nsb);
       const real c = r0.c, p = r0.p, mu = r0.m, lmu = r0.l;
       const real drho = d.drho;
       const real CS2 = (real)(1.0/3.0), ALPHA = (real)3.0, GAMMA_C = (real)1.5, BETA_C = (real)4.5;
