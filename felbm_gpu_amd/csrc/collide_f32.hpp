@@ -11,7 +11,10 @@
 //  - the MRT transform (:617-638) splits each pair into even (y_k + y_opp) and
 //    odd (y_k - y_opp) parts, and Minv = M^T diag(1/|row|^2) folds into the
 //    relaxation rates;
-//  - divisions use the hardware reciprocal.
+//  - divisions use the hardware reciprocal;
+//  - g and p are deviations from the rest pressure (g_k - w_k, p - 1; see
+//    p_ref in kernels.hpp): eq_g and F_g are affine in (g, p) with the w_k
+//    offset cancelling, so the arithmetic below yields the stored deviation.
 #pragma once
 #include "kernels.hpp"
 
@@ -65,9 +68,9 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       #pragma unroll
       for (int k = 0; k < Q; ++k) {
         float hv, gv;
-        plain_eq<float>(k, oux, ouy, ouz, op, orho, oc, hv, gv);
+        plain_eq<float>(k, oux, ouy, ouz, op, orho, oc, hv, gv);     // op = p - 1: gv = g_eq - w_k
         store_post<float, HBB, OPEN>(hv, rH, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
-        store_post<float, HBB, OPEN>(gv, rGo, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+        store_post<float, HBB, OPEN>(g_flush(gv), rGo, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb, -W<float>(k));
       }
       if (stats) {
         double vx = (double)oux, vy = (double)ouy, vz = (double)ouz;
@@ -172,13 +175,20 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       const float u_fg_f = ux*f_fgx + uy*f_fgy + uz*f_fgz;
       // h_k = G_k (K0 + a2 bd_c + a4 bd_p + a5 e_k.bf + mob avg_k)
       //     = G_k (K0h + Ph_k + a5 e_k.bf),  K0h = K0 + mob/2 lap(mu)
-      const float K0h = c + 0.5f*u_term_h - u_fh_f + hmob*lmu;
+      // c is kept apart from the small terms s0h = K0h - c and Ph_k: summed with
+      // c first, terms below ulp(c) (6e-8 near c = 1) would round away, a bias
+      // in the mass budget that the mass correction then moves onto the interface
+      const float s0h = 0.5f*u_term_h - u_fh_f + hmob*lmu;
       // eq_g = w p + wut (B0 - Bc cd_c) + G (u_force_g/2 +- (fp + mu cd_c)/2)
       // F_g  = wut (Bc (cd_c + bd_c) - u_drm) + G (mu/2 (cd_c + bd_c) + fp - u_fg_f)
       const float B0 = rho_cs2 + 0.5f*u_drho, Bc = 0.5f*drho_cs2, hmu = 0.5f*mu, hufg = 0.5f*u_force_g;
       const float gamma_u_sq = 1.5f*u_sq;
       auto storek = [&](float v, rsrc_t r, int k) {
         store_post<float, HBB, OPEN>(v, r, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb);
+      };
+      // g slots hold g_k - w_k; a zero population is stored as -w_k
+      auto storeg = [&](float v, int k) {
+        store_post<float, HBB, OPEN>(g_flush(v), rGo, k, vo[k], (fl >> k) & 1u, (so >> k) & 1u, io, nsb, -W<float>(k));
       };
       float rr[Q], yv[Q];
       float relx = 0.0f;
@@ -194,15 +204,15 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
         const float wut = w*u_term;
         const float G = w + wut;
         // Ph_k carries mob/2 lap(mu)(x+e_k); a non-site neighbour fetched lmu itself
-        const float A = K0h + Ph[k] + a5*fp;
-        storek(G*A, rH, k);
+        // one rounding at ulp(h_k): G c + G (s0h + Ph_k + a5 e_k.bf)
+        storek(__builtin_fmaf(G, c, G*(s0h + Ph[k] + a5*fp)), rH, k);
         const float cdk = cdc[k], dc2k = dc2[k];
         // BGK eq_g carries -(mu cd_c + fp)/2 (device_engine.cuh:516), MRT +(...)/2 (:598)
         const float eg = MRT ? w*p + wut*(B0 - Bc*cdk) + G*(0.5f*fp + hufg + hmu*cdk)
                              : w*p + wut*(B0 - Bc*cdk) + G*(hufg - 0.5f*fp - hmu*cdk);
         const float fg = wut*(Bc*dc2k - u_drm) + G*(hmu*dc2k + fp - u_fg_f);
         if constexpr (!MRT) {
-          storek(gk[k] + relx*(eg - gk[k]) + fg, rGo, k);
+          storeg(gk[k] + relx*(eg - gk[k]) + fg, k);
         } else {
           yv[k] = eg - gk[k];
           rr[k] = gk[k] + fg;
@@ -232,7 +242,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
           #pragma unroll
           for (int m = 0; m < Q; ++m)
             if (!CONS_ROW[m] && EVEN_ROW[m] && MRTTAB.M[m][0] != 0.0) a0 += (float)MRTTAB.M[m][0]*xm[m];
-          storek(rr[0] + a0, rGo, 0);
+          storeg(rr[0] + a0, 0);
         }
         #pragma unroll
         for (int k = 1; k < Q; k += 2) {
@@ -243,8 +253,8 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
             if (EVEN_ROW[m]) E += (float)MRTTAB.M[m][k]*xm[m];
             else O += (float)MRTTAB.M[m][k]*xm[m];
           }
-          storek(rr[k] + (E + O), rGo, k);
-          storek(rr[k + 1] + (E - O), rGo, k + 1);
+          storeg(rr[k] + (E + O), k);
+          storeg(rr[k + 1] + (E - O), k + 1);
         }
       }
     }

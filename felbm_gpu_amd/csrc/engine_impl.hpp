@@ -207,7 +207,14 @@ public:
     // next step's gather reproduces it (k_unstream inverts the gather)
     for (int pass = 0; pass < 2; ++pass) {
       const double *src = pass ? g : h;
-      for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) t[k * ns + i] = (real)src[k * n + i];
+      if (pass && p_ref<real>() != 0.0) {        // FP32 g slots hold g_k - w_k (p_ref, kernels.hpp)
+        for (int k = 0; k < 19; ++k) {
+          const double off = (double)wf32(k) * p_ref<real>();
+          for (size_t i = 0; i < n; ++i) t[k * ns + i] = (real)g_flush((float)(src[k * n + i] - off));
+        }
+      } else {
+        for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) t[k * ns + i] = (real)src[k * n + i];
+      }
       real *dst = pass ? (cur ? S.d.g1 : S.d.g0) : S.d.h;
       FM_HIP(hipMemcpyAsync(stage, t.data(), t.size() * sizeof(real), hipMemcpyHostToDevice, stream_));
       hipLaunchKernelGGL(k_unstream<real>, dim3(nb), dim3(256), 0, stream_, S.d, (const real *)stage, dst);
@@ -249,7 +256,10 @@ public:
     FM_LAUNCH();
     FM_HIP(hipMemcpyAsync(t.data(), stage, t.size() * sizeof(real), hipMemcpyDeviceToHost, stream_));
     FM_HIP(hipStreamSynchronize(stream_));
-    for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) g[k * n + i] = (double)t[k * ns + i];
+    for (int k = 0; k < 19; ++k) {
+      const double off = p_ref<real>() != 0.0 ? (double)wf32(k) * p_ref<real>() : 0.0;
+      for (size_t i = 0; i < n; ++i) g[k * n + i] = (double)t[k * ns + i] + off;
+    }
     return FELBM_OK;
   }
 
@@ -379,7 +389,7 @@ public:
     for (size_t i = 0; i < n; ++i) {
       out[i] = (double)t[i]; out[n + i] = (double)cp[2 * i];
       out[2 * n + i] = (double)t[ns + i]; out[3 * n + i] = (double)t[2 * ns + i]; out[4 * n + i] = (double)t[3 * ns + i];
-      out[5 * n + i] = (double)cp[2 * i + 1];
+      out[5 * n + i] = (double)cp[2 * i + 1] + p_ref<real>();    // p* is stored as p - p_ref
     }
     return FELBM_OK;
   }
@@ -446,7 +456,8 @@ public:
     d.use_in_vel = ob->use_in_vel; d.use_in_prs = ob->use_in_prs; d.use_in_fluid = ob->use_in_fluid;
     d.use_out_prs = ob->use_out_prs; d.use_out_fluid = ob->use_out_fluid; d.inlet_mode = ob->inlet_mode;
     for (int a = 0; a < 3; ++a) d.u_in[a] = ob->u_in[a];
-    d.p_in = ob->p_in; d.p_out = ob->p_out; d.c_out_fixed = ob->c_out_fixed; d.rho_out_fixed = ob->rho_out_fixed;
+    d.p_in = ob->p_in - p_ref<real>(); d.p_out = ob->p_out - p_ref<real>();   // as the state stores pressure
+    d.c_out_fixed = ob->c_out_fixed; d.rho_out_fixed = ob->rho_out_fixed;
     d.inlet_c_a = ob->inlet_c_a; d.inlet_period = ob->inlet_period; d.inlet_duty = ob->inlet_duty;
     d.inlet_ramp = ob->inlet_ramp; d.split_pos = ob->split_pos;
     // the injection schedule starts at step 0 (open_step, multiphase_gpu.cuh:445)

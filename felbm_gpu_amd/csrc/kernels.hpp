@@ -548,6 +548,22 @@ template <typename real> __device__ __forceinline__ real W(int k) { return (real
 // 1/36 its half, and the rest weight the exact remainder 1 - 12 w1.
 constexpr float WF0 = 0x1.555558p-2f, WF1 = 0x1.c71c70p-5f, WF2 = 0x1.c71c70p-6f;
 template <> __device__ __forceinline__ float W<float>(int k) { return k == 0 ? WF0 : (k < 7 ? WF1 : WF2); }
+__host__ __device__ constexpr float wf32(int k) { return k == 0 ? WF0 : (k < 7 ? WF1 : WF2); }
+
+// Reference pressure of the stored state.  The FP32 state carries the
+// pressure as a deviation from the rest pressure 1: the p fields hold p - 1 and
+// g slot k holds g_k - w_k (FP32 weights; the zero population is stored as
+// -w_k).  An absolute FP32 pressure near 1 rounds at 6e-8, and that rounding,
+// fed through the pressure gradients of the h forcing, was the largest FP32
+// error in c; the deviations round 10^2-10^3 times finer.  Every equilibrium
+// and forcing term is affine in (g, p) with the offset w_k p_ref cancelling, so
+// the passes compute the deviations directly.  The FP64 state is absolute.
+template <typename real> __host__ __device__ constexpr double p_ref() { return sizeof(real) == 4 ? 1.0 : 0.0; }
+// FP32 g deviations below 2^-31 are stored as 0 (below the resolution of an
+// absolute FP32 g), so the exchanged double state (w_k + s, exact in double)
+// converts back to the same FP32 bits
+constexpr float G_FLUSH = 0x1p-31f;
+__host__ __device__ inline float g_flush(float s) { return (s < G_FLUSH && s > -G_FLUSH) ? 0.0f : s; }
 
 // ---- mass-correction scalar from the partial sums of this step ------------------
 // lambda = (M - M0)/W applied when W > 0 and M != M0 (multiphase_gpu.cuh:689-700)
@@ -957,16 +973,17 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
 // its own slot k (contiguous across the wave); the next gather reads it from
 // x + e_k, or -- neighbour not a site -- from slot opp(k) of x as the
 // bounced-back population (halfway bounce-back off a solid) or zero (outside
-// the box, or no bounce-back).  vo is unused: the gather side resolves it.
+// the box, or no bounce-back; `zero` is the stored form of a zero population).
+// vo is unused: the gather side resolves it.
 // The FP32 store carries the non-temporal hint: nothing reads the slot before
 // the next step, and the line need not displace the stencil data in L2.
 template <typename real, bool HBB, bool OPEN>
 __device__ __forceinline__ void store_post(real v, rsrc_t r, int k, unsigned vo, bool fluid, bool solid,
-                                     unsigned io, unsigned nsb)
+                                     unsigned io, unsigned nsb, real zero = (real)0)
 {
   (void)vo;
   real val = v;
-  if constexpr (OPEN || !HBB) val = (fluid || (HBB && solid)) ? v : (real)0;
+  if constexpr (OPEN || !HBB) val = (fluid || (HBB && solid)) ? v : zero;
   (void)solid;
   if constexpr (sizeof(real) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, val), r, (int)io, (int)((unsigned)k*nsb), 2);
   else bst(val, r, io, (unsigned)k*nsb);

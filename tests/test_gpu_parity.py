@@ -154,25 +154,28 @@ def _fp32_errors(o, g, steps):
     return dc, du
 
 
-# FP32 tolerance against the FP64 oracle: 1e-5 in c and u over 20 and over
-# 200 steps (measured: 1.1e-6 and 3.1e-6 in c, 1.2e-7 in u).
+# FP32 tolerance against the FP64 oracle: 1e-5 in c over 20 and over 200
+# steps, and 5e-8 in u (measured: 7.8e-7 and 1.2e-6 in c, 6e-9 and 8e-9 in u).
+# The u bound holds only with the pressure stored as a deviation from the rest
+# pressure (p - 1, g_k - w_k): with absolute FP32 pressures u was off by 1.2e-7
+# after 200 steps (and c by 3.1e-6).
 def test_fp32_within_tolerance_of_fp64_oracle():
     solid = small_pack(24, 0.55, 11)
     prm = W.derive_params(theta_deg=45.0, acceleration=(1e-5, 0, 0))
     o, g = make_pair(solid, prm, 1, 1, precision="f32")
     dc, du = _fp32_errors(o, g, 20)
     _say("fp32 mrt 20 steps max|dc|", dc, "max|du|", du)
-    assert dc <= 1e-5 and du <= 1e-5
+    assert dc <= 1e-5 and du <= 5e-8
     dc, du = _fp32_errors(o, g, 180)
     _say("fp32 mrt 200 steps max|dc|", dc, "max|du|", du)
-    assert du <= 1e-5 and dc <= 1e-5
+    assert du <= 5e-8 and dc <= 1e-5
 
 
 # Without the mass correction, the FP32 run's total c may differ from the FP64
 # run's only by rounding: lattice weights whose FP32 values sum to 1 + 2^-26
 # (1/3, 1/18, 1/36 rounded to nearest) add 1.5e-8 of the mass every step
 # (measured 1.6e-8 per step here); the engine's weights sum to exactly 1
-# (measured 1.6e-9 per step).
+# (measured 1.6e-9 per step; 1.3e-9 with h_k rounded once, G_k c + G_k s_k).
 def test_fp32_mass_drift_below_weight_rounding():
     edge, steps = 64, 400
     sol = W.spheres(edge, edge, edge, 5.0, 9.0, 0.4, 1000)
@@ -253,10 +256,10 @@ def test_fp32_bgk_within_tolerance():
     o, g = make_pair(w["solid"], w["params"], 0, 1, precision="f32", c0=w["c0"])
     dc, du = _fp32_errors(o, g, 20)
     _say("fp32 bgk 20 steps max|dc|", dc, "max|du|", du)
-    assert dc <= 1e-5 and du <= 1e-5
+    assert dc <= 1e-5 and du <= 1e-7
     dc, du = _fp32_errors(o, g, 180)
     _say("fp32 bgk 200 steps max|dc|", dc, "max|du|", du)
-    assert du <= 1e-5 and dc <= 2e-4
+    assert du <= 1e-7 and dc <= 2e-4
 
 
 @pytest.mark.parametrize("use_mrt", [0, 1])
