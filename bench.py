@@ -166,6 +166,8 @@ def main():
         f32_err = {"steps": args.err_steps}
         for k in ("rho", "c", "ux", "uy", "uz", "p"):
             f32_err[k] = float(np.abs(fields["f32"][k] - fields["f64"][k]).max())
+        dc = np.abs(fields["f32"]["c"] - fields["f64"]["c"])
+        f32_err["c_p99"], f32_err["c_median"] = float(np.percentile(dc, 99)), float(np.median(dc))
         del fields
         log("fp32 vs fp64 after %d steps: %s" % (args.err_steps, json.dumps(f32_err)))
 

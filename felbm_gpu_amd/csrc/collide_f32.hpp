@@ -204,8 +204,12 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
         const float wut = w*u_term;
         const float G = w + wut;
         // Ph_k carries mob/2 lap(mu)(x+e_k); a non-site neighbour fetched lmu itself
-        // one rounding at ulp(h_k): G c + G (s0h + Ph_k + a5 e_k.bf)
-        storek(__builtin_fmaf(G, c, G*(s0h + Ph[k] + a5*fp)), rH, k);
+        // one rounding at ulp(h_k): w c + wut c + G (s0h + Ph_k + a5 e_k.bf).
+        // G = w + wut is not rounded on this path: the rest population's
+        // G_0 = w_0 (1 - 1.5 u^2) differs from w_0 by a fixed fraction of an
+        // ulp at every site of a given |u|, and rounding it first biased the
+        // mass of every bulk site the same way (+5e-9 per site at |u| = 1e-4)
+        storek(__builtin_fmaf(w, c, __builtin_fmaf(wut, c, G*(s0h + Ph[k] + a5*fp))), rH, k);
         const float cdk = cdc[k], dc2k = dc2[k];
         // BGK eq_g carries -(mu cd_c + fp)/2 (device_engine.cuh:516), MRT +(...)/2 (:598)
         const float eg = MRT ? w*p + wut*(B0 - Bc*cdk) + G*(0.5f*fp + hufg + hmu*cdk)
