@@ -175,7 +175,7 @@ def main():
     if args.sweep_steps > 0:
         eng.free()
         from felbm_gpu_amd import serving
-        sweep = serving.run_sweep(W.sweep_instances(48, 1000), args.sweep_steps, "f32", log=log)
+        sweep = serving.run_sweep(W.sweep_instances(48, 1000), args.sweep_steps, "f32", log=log, check_fp64=True)
         log("sweep: %s" % json.dumps(sweep))
     served = None
     if args.serve > 0:

@@ -39,6 +39,13 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
   const bool live = i < d.n;
   Fx s0 = {0,0,0}, s1 = {0,0,0}, s2 = {0,0,0}, s3 = {0,0,0}, s4 = {0,0,0};
   unsigned long long vmax = 0;
+  // this step's mass-correction lambda (k_mass_lambda), loaded before the
+  // stencil loads so that waiting for it does not wait for them
+  float lam = 0.0f;
+  if (d.mass_on) {
+    const unsigned long long ap = ldg(d.acc, ACC_FLAGS + 4), lb = ldg(d.acc, ACC_FLAGS + 3);
+    lam = ap ? (float)__longlong_as_double((long long)lb) : 0.0f;
+  }
   if (threadIdx.x == 0 && bm.y == 0) {
     if (d.mass_on) stg(d.acc, ACC_FLAGS, 1ull);
     if (OPEN && d.obc) stg(d.acc, ACC_FLAGS + 1, ldg(d.acc, ACC_FLAGS + 1) + 1ull);   // ++open_step
@@ -81,6 +88,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
     } else {
       float cN[Q], pN[Q], c2[Q], p2[Q], lN[Q];
       const Rec4<float> r0 = bld_rec4((const float *)nullptr, rF, io, F_REC*nsb);
+      const float craw = bld_nt<float>(rF, io, F_CRAW*nsb);      // read once
       {
         // a neighbour that is not a site takes the own record's values
         unsigned vo2[Q];
@@ -95,6 +103,15 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
         }
       }
       const float c = r0.c, p = r0.p, mu = r0.m, lmu = r0.l;
+      // The mass correction c = c_raw - lambda phi+ (applied by the c pass) is
+      // an FP32 value, and near the interface the correction per step (1e-7 to
+      // 1e-6) is carried only to half an ulp of c (3e-8 at c = 0.5): the same
+      // rounding at the same sites every step, 2e-4 in c after 1000 steps at a
+      // 139-degree contact angle.  What it dropped ((c_raw - c) is exact) joins
+      // the small terms of h, so the populations carry the correction, and
+      // stream it, as the FP64 state does.  0 when no correction applies.
+      const float phi_raw = craw*(1.0f - craw);        // as corrected() forms it
+      const float cres = (craw - c) - lam*(phi_raw > 0.0f ? phi_raw : 0.0f);
       const float drho = d.drho;
       const float CS2 = 1.0f/3.0f;
       const float rho = ovp ? ldg(ovp, 1) : d.rho1 + drho*c;
@@ -178,7 +195,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       // c is kept apart from the small terms s0h = K0h - c and Ph_k: summed with
       // c first, terms below ulp(c) (6e-8 near c = 1) would round away, a bias
       // in the mass budget that the mass correction then moves onto the interface
-      const float s0h = 0.5f*u_term_h - u_fh_f + hmob*lmu;
+      const float s0h = 0.5f*u_term_h - u_fh_f + hmob*lmu + cres;
       // eq_g = w p + wut (B0 - Bc cd_c) + G (u_force_g/2 +- (fp + mu cd_c)/2)
       // F_g  = wut (Bc (cd_c + bd_c) - u_drm) + G (mu/2 (cd_c + bd_c) + fp - u_fg_f)
       const float B0 = rho_cs2 + 0.5f*u_drho, Bc = 0.5f*drho_cs2, hmu = 0.5f*mu, hufg = 0.5f*u_force_g;

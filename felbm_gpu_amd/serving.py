@@ -45,9 +45,32 @@ def sweep_problem(spec):
     return solid, prm, W.slab_c(solid, 0, 0, edge // 2, 4.0)
 
 
-def run_sweep(specs, steps, precision="f32", graph_chunk=50, log=None):
+def _batch_fields(problems, precision, steps, graph_chunk):
+    """Fields of every instance after 2 + steps batched steps (graph replay)."""
+    from .engine import Batch, MultiPhaseGPU
+    sw = {"use_mrt": 1, "correct_op_mass": 1}
+    b = Batch(precision=precision)
+    insts = []
+    for solid, prm, c0 in problems:
+        g = MultiPhaseGPU(batch=b).init(solid, prm, sw)
+        g.init_rest(c0, 1.0)
+        g.record_target_mass()
+        insts.append(g)
+    b.step(2)
+    b.step_graph(graph_chunk, steps // graph_chunk)
+    b.synchronize()
+    out = [g.download() for g in insts]
+    for g in insts:
+        g.free()
+    b.free()
+    return out
+
+
+def run_sweep(specs, steps, precision="f32", graph_chunk=50, log=None, check_fp64=False):
     """Batched vs sequential MLUPS over the same instances (MRT, mass
-    correction on, FP32 distributions with FP64 sums)."""
+    correction on, FP32 distributions with FP64 sums).  check_fp64 also runs
+    the batch in FP64 and reports the largest FP32 - FP64 difference of c and
+    u over all instances after the same steps."""
     from .engine import Batch, MultiPhaseGPU
     sw = {"use_mrt": 1, "correct_op_mass": 1}
     t_gen = time.time()
@@ -94,9 +117,27 @@ def run_sweep(specs, steps, precision="f32", graph_chunk=50, log=None):
     if log:
         log("sweep: sequential %.3fs" % t_seq)
     lu = float(total) * steps
-    return {"instances": len(problems), "sites": int(total), "steps": steps,
-            "batched_MLUPS": lu / t_batched / 1e6, "sequential_MLUPS": lu / t_seq / 1e6,
-            "speedup": t_seq / t_batched, "batched_s": t_batched, "sequential_s": t_seq}
+    res = {"instances": len(problems), "sites": int(total), "steps": steps,
+           "batched_MLUPS": lu / t_batched / 1e6, "sequential_MLUPS": lu / t_seq / 1e6,
+           "speedup": t_seq / t_batched, "batched_s": t_batched, "sequential_s": t_seq}
+    if check_fp64 and precision == "f32":
+        f32 = _batch_fields(problems, "f32", steps, graph_chunk)
+        f64 = _batch_fields(problems, "f64", steps, graph_chunk)
+        finite = lambda f: all(np.isfinite(f[k]).all() for k in f)
+        # an instance whose FP64 run is not finite (the parameters leave the
+        # method's stable range) has no reference to compare against
+        pairs = [(a, b) for a, b in zip(f32, f64) if finite(b)]
+        dc = [float(np.abs(a["c"] - b["c"]).max()) for a, b in pairs]
+        du = [max(float(np.abs(a[k] - b[k]).max()) for k in ("ux", "uy", "uz")) for a, b in pairs]
+        res["fp32_vs_fp64"] = {"steps": steps + 2, "compared": len(pairs),
+                               "fp64_nonfinite": len(problems) - len(pairs),
+                               "fp32_nonfinite_where_fp64_finite": int(sum(not finite(a) for a, _ in pairs)),
+                               "max_dc": max(dc) if dc else None, "max_du": max(du) if du else None,
+                               "median_instance_dc": float(np.median(dc)) if dc else None,
+                               "instances_dc_above_1e-5": int(sum(v > 1e-5 for v in dc))}
+        if log:
+            log("sweep: fp32 vs fp64 %s" % res["fp32_vs_fp64"])
+    return res
 
 
 # ---- served request stream ----------------------------------------------------------
