@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--serve", type=int, default=16, help="requests of the served stream (0 skips it)")
     ap.add_argument("--err-steps", type=int, default=1000,
                     help="steps of the FP32-vs-FP64 field comparison on the same lattice (0 skips it)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep 256 sites per workgroup in every pass instead of the autotuned shapes")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="only run the device-to-device copy probe of this size (counter calibration)")
     args = ap.parse_args()
@@ -104,6 +106,11 @@ def main():
     n = eng.n
     print("[bench] sites %d, precision %s" % (n, w["precision"]), file=sys.stderr, flush=True)
 
+    tune = None
+    if not args.no_autotune:
+        # workgroup shape per pass, timed on this lattice; the state is left as it was
+        tune = eng.batch.autotune(10)
+        print("[bench] autotuned sites per workgroup %s" % json.dumps(tune), file=sys.stderr, flush=True)
     eng.step(args.warmup)
     eng.batch.synchronize()
     t0 = time.perf_counter()
@@ -202,6 +209,7 @@ def main():
                    "correct_op_mass": bool(w["switches"]["correct_op_mass"]),
                    "hbm_frac_total": mlups * 1e6 * B_ALG_PER_LU / (HBM_PEAK_GBS * 1e9)},
         "kernels_ms": {k: v["ms"] for k, v in kern.items()},
+        "autotune": tune,
         "mass_drift": mass_drift,
         "with_tracers_MLUPS": tr_mlups,
         "fp32_max_abs_diff_vs_fp64": f32_err,

@@ -89,6 +89,8 @@ SIGNATURES = {
     "felbm_export_tables": (_I, [_P, _I] + [_PI32] * 10),
     "felbm_mrt_matrices": (_I, [_PD, _PD]),
     "felbm_kernel_times": (_I, [_P, _I, C.POINTER(C.c_float)]),
+    "felbm_autotune": (_I, [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+    "felbm_set_block_sizes": (_I, [_P, C.POINTER(C.c_int)]),
     "felbm_copy_probe": (_I, [_I, C.c_int64, _I, _PD]),
     "felbm_last_error": (C.c_char_p, []),
     "felbm_version": (C.c_char_p, []),

@@ -112,6 +112,8 @@ int felbm_export_tables(felbm_engine *e, int slot, int32_t *cdm, int32_t *cdp, i
   return e->impl->export_tables(slot, t);
 }
 int felbm_kernel_times(felbm_engine *e, int nsteps, float *ms4) { ENG(e); if (!ms4) return FELBM_EINVAL; return e->impl->kernel_times(nsteps, ms4); }
+int felbm_autotune(felbm_engine *e, int nsteps, int *bs4, float *ms12) { ENG(e); return e->impl->autotune(nsteps, bs4, ms12); }
+int felbm_set_block_sizes(felbm_engine *e, const int *bs4) { ENG(e); if (!bs4) return FELBM_EINVAL; return e->impl->set_block_sizes(bs4); }
 
 int felbm_mrt_matrices(double *M, double *Minv)
 {

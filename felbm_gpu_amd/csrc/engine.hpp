@@ -35,6 +35,8 @@ struct EngineBase {
                            const int32_t *bv, const int32_t *br, int n_buf, int copy_to_buffers) = 0;
   virtual int export_tables(int slot, int32_t **tabs) = 0;
   virtual int kernel_times(int nsteps, float *ms4) = 0;
+  virtual int autotune(int nsteps, int *bs4, float *ms12) = 0;
+  virtual int set_block_sizes(const int *bs4) = 0;
   virtual int tracers_init(int slot, int64_t np, const double *xyz, const double *dm, uint64_t seed, int substeps,
                            int vskip) = 0;
   virtual int tracers_download(int slot, double *xyz, double *uvw, int32_t *blocked) = 0;

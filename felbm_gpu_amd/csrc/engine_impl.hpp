@@ -662,6 +662,71 @@ public:
     return FELBM_OK;
   }
 
+  // Tile-shape autotuner: every pass timed at each workgroup shape (HIP
+  // events around its launches, after two warm-up steps per shape) on the
+  // instances present; the fastest shape per pass is kept.  The state of
+  // every instance is copied aside first and restored afterwards, so the
+  // tuning steps leave no trace.  Not with tracers (their positions live
+  // outside the instance allocation).
+  int autotune(int nsteps, int *bs4, float *ms12) override {
+    if (nsteps <= 0) { set_error("autotune needs at least one timed step"); return FELBM_EINVAL; }
+    if (int e = flush()) return e;
+    for (auto &s : slots_)
+      if (s.used && s.tr_np) { set_error("autotune before tracers are added (the tuning steps would move them)"); return FELBM_ESTATE; }
+    float ms[3][4] = {};
+    if (nmap_ > 0) {
+      std::vector<std::pair<Slot<real> *, char *>> saved;
+      auto drop = [&]() { for (auto &x : saved) (void)hipFree(x.second); saved.clear(); };
+      for (auto &s : slots_) {
+        if (!s.used) continue;
+        char *p = nullptr;
+        if (hipMalloc((void **)&p, s.bytes) != hipSuccess) { drop(); set_error("device allocation failed (autotune state copy)"); return FELBM_ENOMEM; }
+        saved.push_back({&s, p});
+        if (hipMemcpyAsync(p, s.base, s.bytes, hipMemcpyDeviceToDevice, stream_) != hipSuccess) {
+          drop(); set_error("state copy failed (autotune)"); return FELBM_EHIP;
+        }
+      }
+      const unsigned T0 = T_;
+      int keep[5]; std::copy(bsi_, bsi_ + 5, keep);
+      int rc = FELBM_OK;
+      for (int bi = 0; bi < 3 && rc == FELBM_OK; ++bi) {
+        for (int q = 1; q <= 4; ++q) bsi_[q] = bi;
+        for (int w = 0; w < 2 && rc == FELBM_OK; ++w) rc = launch_step(false);
+        if (rc == FELBM_OK) rc = kernel_times(nsteps, ms[bi]);
+      }
+      for (auto &x : saved)
+        if (rc == FELBM_OK && hipMemcpyAsync(x.first->base, x.second, x.first->bytes, hipMemcpyDeviceToDevice, stream_) != hipSuccess) {
+          set_error("state restore failed (autotune)"); rc = FELBM_EHIP;
+        }
+      if (hipStreamSynchronize(stream_) != hipSuccess && rc == FELBM_OK) { set_error("autotune synchronisation failed"); rc = FELBM_EHIP; }
+      drop();
+      T_ = T0;
+      for (auto &s : slots_) if (s.used) s.stats_valid = 0;
+      ++map_ver_;                     // a captured step graph has the old shapes
+      if (rc != FELBM_OK) { std::copy(keep, keep + 5, bsi_); return rc; }
+      for (int q = 1; q <= 4; ++q) {
+        int best = 2;
+        for (int bi = 0; bi < 3; ++bi) if (ms[bi][q - 1] < ms[best][q - 1]) best = bi;
+        bsi_[q] = best;
+      }
+    }
+    if (bs4) for (int q = 0; q < 4; ++q) bs4[q] = kBS[bsi_[q + 1]];
+    if (ms12) for (int bi = 0; bi < 3; ++bi) for (int q = 0; q < 4; ++q) ms12[bi * 4 + q] = ms[bi][q];
+    return FELBM_OK;
+  }
+
+  int set_block_sizes(const int *bs4) override {
+    int idx[4];
+    for (int q = 0; q < 4; ++q) {
+      idx[q] = -1;
+      for (int bi = 0; bi < 3; ++bi) if (bs4[q] == kBS[bi]) idx[q] = bi;
+      if (idx[q] < 0) { set_error("sites per workgroup must be 64, 128 or 256"); return FELBM_EINVAL; }
+    }
+    for (int q = 0; q < 4; ++q) bsi_[q + 1] = idx[q];
+    ++map_ver_;
+    return FELBM_OK;
+  }
+
 private:
   static constexpr int kMaxAsyncSlots = 256;
   int dev_;
@@ -671,8 +736,11 @@ private:
   std::vector<int> slot_desc_;      // slot -> descriptor index
   Inst<real> *d_desc_ = nullptr; int desc_cap_ = 0, ndesc_ = 0;
   int2 *d_map_ = nullptr;
-  int2 *d_var_[8] = {};
-  int nvar_[8] = {};
+  // block maps at each workgroup shape (sites per workgroup), per variant
+  static constexpr int kBS[3] = { 64, 128, 256 };
+  int2 *d_var_[3][8] = {};
+  int nvar_[3][8] = {};
+  int bsi_[5] = { 2, 2, 2, 2, 2 };  // shape index of passes 1-4 (moments, c, mu, collide)
   int map_cap_ = 0;
   int nmap_ = 0;
   static int variant(const Slot<real> &s) { return (s.mrt ? 1 : 0) | (s.hbb ? 2 : 0) | ((s.open || s.obc) ? 4 : 0); }
@@ -726,24 +794,23 @@ private:
   int rebuild() {
     slot_desc_.assign(slots_.size(), -1);
     std::vector<Inst<real>> descs;
-    std::vector<int2> all, var[8];
+    std::vector<int2> all, var[3][8];
     for (size_t q = 0; q < slots_.size(); ++q) {
       if (!slots_[q].used) continue;
       int di = (int)descs.size();
       slot_desc_[q] = di;
       descs.push_back(slots_[q].d);
       const int key = variant(slots_[q]);
-      for (int b = 0; b < slots_[q].nblocks; ++b) {
-        int2 e = make_int2(di, b * 256);
-        all.push_back(e);
-        var[key].push_back(e);
-      }
+      for (int b = 0; b < slots_[q].nblocks; ++b) all.push_back(make_int2(di, b * 256));
+      for (int bi = 0; bi < 3; ++bi)
+        for (int b = 0; b * kBS[bi] < slots_[q].n; ++b) var[bi][key].push_back(make_int2(di, b * kBS[bi]));
     }
     auto pad = [](std::vector<int2> &v) { while (v.size() % 8) v.push_back(make_int2(-1, 0)); };
-    pad(all); for (auto &v : var) pad(v);
+    pad(all); for (auto &vb : var) for (auto &v : vb) pad(v);
     nmap_ = (int)all.size();
     int need = nmap_;
-    for (int k = 0; k < 8; ++k) { nvar_[k] = (int)var[k].size(); need += nvar_[k]; }
+    for (int bi = 0; bi < 3; ++bi)
+      for (int k = 0; k < 8; ++k) { nvar_[bi][k] = (int)var[bi][k].size(); need += nvar_[bi][k]; }
     if ((int)descs.size() > desc_cap_) {
       if (d_desc_) FM_HIP(hipFree(d_desc_));
       desc_cap_ = std::max((int)descs.size(), 2 * desc_cap_);
@@ -759,11 +826,13 @@ private:
     if (nmap_) FM_HIP(hipMemcpyAsync(d_map_, all.data(), sizeof(int2) * nmap_, hipMemcpyHostToDevice, stream_));
     {
       int2 *p = d_map_ + nmap_;
-      for (int k = 0; k < 8; ++k) {
-        d_var_[k] = p;
-        if (nvar_[k]) FM_HIP(hipMemcpyAsync(p, var[k].data(), sizeof(int2) * nvar_[k], hipMemcpyHostToDevice, stream_));
-        p += nvar_[k];
-      }
+      for (int bi = 0; bi < 3; ++bi)
+        for (int k = 0; k < 8; ++k) {
+          d_var_[bi][k] = p;
+          if (nvar_[bi][k])
+            FM_HIP(hipMemcpyAsync(p, var[bi][k].data(), sizeof(int2) * nvar_[bi][k], hipMemcpyHostToDevice, stream_));
+          p += nvar_[bi][k];
+        }
     }
     // open-boundary nodes of every slot, one list for one launch per step
     {
@@ -816,43 +885,45 @@ private:
   }
 
   template <bool MRT, bool HBB, bool OPEN>
-  void launch_collide(dim3 grid, const int2 *map, bool stats) {
+  void launch_collide(dim3 grid, const int2 *map, bool stats, int bs) {
     if constexpr (sizeof(real) == 4)
-      hipLaunchKernelGGL((k_collide_f32<MRT, HBB, OPEN>), grid, dim3(256), 0, stream_, (const Inst<float> *)d_desc_, map, T_, stats ? 1 : 0);
+      hipLaunchKernelGGL((k_collide_f32<MRT, HBB, OPEN>), grid, dim3(bs), 0, stream_, (const Inst<float> *)d_desc_, map, T_, stats ? 1 : 0);
     else
-      hipLaunchKernelGGL((k_collide<real, MRT, HBB, OPEN>), grid, dim3(256), 0, stream_, d_desc_, map, T_, stats ? 1 : 0);
+      hipLaunchKernelGGL((k_collide<real, MRT, HBB, OPEN>), grid, dim3(bs), 0, stream_, d_desc_, map, T_, stats ? 1 : 0);
   }
   template <bool MRT, bool HBB, bool OPEN>
-  void launch_variant(int stage, dim3 grid, const int2 *map, bool stats, real *exp, int exp_inst) {
-    if (stage == 2) hipLaunchKernelGGL((k_cfield<real, HBB, OPEN>), grid, dim3(256), 0, stream_, d_desc_, map, T_, exp, exp_inst);
-    else if (stage == 3) hipLaunchKernelGGL((k_mufield<real, HBB, OPEN>), grid, dim3(256), 0, stream_, d_desc_, map, T_);
-    else launch_collide<MRT, HBB, OPEN>(grid, map, stats);
+  void launch_variant(int stage, dim3 grid, const int2 *map, bool stats, real *exp, int exp_inst, int bs) {
+    if (stage == 2) hipLaunchKernelGGL((k_cfield<real, HBB, OPEN>), grid, dim3(bs), 0, stream_, d_desc_, map, T_, exp, exp_inst);
+    else if (stage == 3) hipLaunchKernelGGL((k_mufield<real, HBB, OPEN>), grid, dim3(bs), 0, stream_, d_desc_, map, T_);
+    else launch_collide<MRT, HBB, OPEN>(grid, map, stats, bs);
   }
-  void launch_key(int key, int stage, dim3 grid, const int2 *map, bool stats, real *exp = nullptr, int exp_inst = -1) {
+  void launch_key(int key, int stage, dim3 grid, const int2 *map, bool stats, real *exp = nullptr, int exp_inst = -1,
+                  int bs = 256) {
     switch (key) {
-      case 0: launch_variant<false, false, false>(stage, grid, map, stats, exp, exp_inst); break;
-      case 1: launch_variant<true,  false, false>(stage, grid, map, stats, exp, exp_inst); break;
-      case 2: launch_variant<false, true,  false>(stage, grid, map, stats, exp, exp_inst); break;
-      case 3: launch_variant<true,  true,  false>(stage, grid, map, stats, exp, exp_inst); break;
-      case 4: launch_variant<false, false, true >(stage, grid, map, stats, exp, exp_inst); break;
-      case 5: launch_variant<true,  false, true >(stage, grid, map, stats, exp, exp_inst); break;
-      case 6: launch_variant<false, true,  true >(stage, grid, map, stats, exp, exp_inst); break;
-      default: launch_variant<true,  true,  true >(stage, grid, map, stats, exp, exp_inst); break;
+      case 0: launch_variant<false, false, false>(stage, grid, map, stats, exp, exp_inst, bs); break;
+      case 1: launch_variant<true,  false, false>(stage, grid, map, stats, exp, exp_inst, bs); break;
+      case 2: launch_variant<false, true,  false>(stage, grid, map, stats, exp, exp_inst, bs); break;
+      case 3: launch_variant<true,  true,  false>(stage, grid, map, stats, exp, exp_inst, bs); break;
+      case 4: launch_variant<false, false, true >(stage, grid, map, stats, exp, exp_inst, bs); break;
+      case 5: launch_variant<true,  false, true >(stage, grid, map, stats, exp, exp_inst, bs); break;
+      case 6: launch_variant<false, true,  true >(stage, grid, map, stats, exp, exp_inst, bs); break;
+      default: launch_variant<true,  true,  true >(stage, grid, map, stats, exp, exp_inst, bs); break;
     }
   }
 
   // the moments pass gathers through the neighbour rings: open (non-periodic)
   // variants need the in-box checks
-  void launch_moments(int key, dim3 grid, const int2 *map) {
-    if (key & 4) hipLaunchKernelGGL((k_moments<real, true>), grid, dim3(256), 0, stream_, d_desc_, map, T_);
-    else hipLaunchKernelGGL((k_moments<real, false>), grid, dim3(256), 0, stream_, d_desc_, map, T_);
+  void launch_moments(int key, dim3 grid, const int2 *map, int bs = 256) {
+    if (key & 4) hipLaunchKernelGGL((k_moments<real, true>), grid, dim3(bs), 0, stream_, d_desc_, map, T_);
+    else hipLaunchKernelGGL((k_moments<real, false>), grid, dim3(bs), 0, stream_, d_desc_, map, T_);
   }
 
   int launch_step(bool stats, bool timed = false) {
     if (stats) if (int e = zero_stats()) return e;
     dim3 blk(256);
     if (timed) FM_HIP(hipEventRecord(ev_[0], stream_));
-    for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_moments(k, dim3(nvar_[k]), d_var_[k]); FM_LAUNCH(); }
+    int bi = bsi_[1];
+    for (int k = 0; k < 8; ++k) if (nvar_[bi][k]) { launch_moments(k, dim3(nvar_[bi][k]), d_var_[bi][k], kBS[bi]); FM_LAUNCH(); }
     if (n_ent_all_) {
       hipLaunchKernelGGL(k_open_fields<real>, dim3((n_ent_all_ + 255) / 256), blk, 0, stream_, d_desc_, d_ent_all_, n_ent_all_, T_);
       FM_LAUNCH();
@@ -864,9 +935,11 @@ private:
       FM_LAUNCH();
     }
     if (timed) FM_HIP(hipEventRecord(ev_[1], stream_));
-    for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_key(k, 2, dim3(nvar_[k]), d_var_[k], stats); FM_LAUNCH(); }
+    bi = bsi_[2];
+    for (int k = 0; k < 8; ++k) if (nvar_[bi][k]) { launch_key(k, 2, dim3(nvar_[bi][k]), d_var_[bi][k], stats, nullptr, -1, kBS[bi]); FM_LAUNCH(); }
     if (timed) FM_HIP(hipEventRecord(ev_[2], stream_));
-    for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_key(k, 3, dim3(nvar_[k]), d_var_[k], stats); FM_LAUNCH(); }
+    bi = bsi_[3];
+    for (int k = 0; k < 8; ++k) if (nvar_[bi][k]) { launch_key(k, 3, dim3(nvar_[bi][k]), d_var_[bi][k], stats, nullptr, -1, kBS[bi]); FM_LAUNCH(); }
     // tracers move on this step's u snapshot before the collision pass
     // advances the step counter they key their draws with
     for (size_t q = 0; q < slots_.size(); ++q) {
@@ -882,7 +955,8 @@ private:
       }
     }
     if (timed) FM_HIP(hipEventRecord(ev_[3], stream_));
-    for (int k = 0; k < 8; ++k) if (nvar_[k]) { launch_key(k, 4, dim3(nvar_[k]), d_var_[k], stats); FM_LAUNCH(); }
+    bi = bsi_[4];
+    for (int k = 0; k < 8; ++k) if (nvar_[bi][k]) { launch_key(k, 4, dim3(nvar_[bi][k]), d_var_[bi][k], stats, nullptr, -1, kBS[bi]); FM_LAUNCH(); }
     if (timed) FM_HIP(hipEventRecord(ev_[4], stream_));
     ++T_;
     return FELBM_OK;

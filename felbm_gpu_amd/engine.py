@@ -98,6 +98,21 @@ class Batch:
         check(self.lib.felbm_kernel_times(self.h, int(nsteps), ms))
         return list(ms)
 
+    def autotune(self, nsteps=10):
+        """Pick the sites per workgroup of each pass (64, 128, 256) by timing
+        them on the instances present; the state is left as it was."""
+        bs = (C.c_int * 4)()
+        ms = (C.c_float * 12)()
+        check(self.lib.felbm_autotune(self.h, int(nsteps), bs, ms))
+        passes = ("moments", "cfield", "mufield", "collide")
+        return {"block_sizes": dict(zip(passes, list(bs))),
+                "ms": {str(b): dict(zip(passes, list(ms)[4 * q:4 * q + 4])) for q, b in enumerate((64, 128, 256))}}
+
+    def set_block_sizes(self, sizes):
+        """Sites per workgroup of moments, c-stencil, mu-stencil, collide."""
+        bs = (C.c_int * 4)(*[int(v) for v in sizes])
+        check(self.lib.felbm_set_block_sizes(self.h, bs))
+
     def free(self):
         if self.h:
             self.lib.felbm_destroy(self.h)

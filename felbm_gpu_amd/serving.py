@@ -3,8 +3,9 @@ request stream.
 
 sweep     N independent instances (edge, porosity, contact angle, mobility
           drawn per instance) advanced together -- one launch per pass serves
-          all of them, the step loop captured in a HIP graph -- against the same
-          instances run one after another through a single-instance engine.
+          all of them, the workgroup shape of each pass autotuned on the batch,
+          the step loop captured in a HIP graph -- against the same instances
+          run one after another through a single-instance engine.
           Batched MLUPS = sum_i n_i steps / wall seconds.
 
 serve     a stream of solve requests with Poisson arrivals.  Each request is an
@@ -88,6 +89,10 @@ def run_sweep(specs, steps, precision="f32", graph_chunk=50, log=None, check_fp6
         g.record_target_mass()
         insts.append(g)
     total = sum(g.n for g in insts)
+    # workgroup shape per pass, timed on this batch (the state is left as it was)
+    tune = b.autotune(10)
+    if log:
+        log("sweep: autotuned sites per workgroup %s" % tune["block_sizes"])
     b.step(2)
     b.synchronize()
     t0 = time.perf_counter()
@@ -119,7 +124,8 @@ def run_sweep(specs, steps, precision="f32", graph_chunk=50, log=None, check_fp6
     lu = float(total) * steps
     res = {"instances": len(problems), "sites": int(total), "steps": steps,
            "batched_MLUPS": lu / t_batched / 1e6, "sequential_MLUPS": lu / t_seq / 1e6,
-           "speedup": t_seq / t_batched, "batched_s": t_batched, "sequential_s": t_seq}
+           "speedup": t_seq / t_batched, "batched_s": t_batched, "sequential_s": t_seq,
+           "autotune": tune}
     if check_fp64 and precision == "f32":
         f32 = _batch_fields(problems, "f32", steps, graph_chunk)
         f64 = _batch_fields(problems, "f64", steps, graph_chunk)

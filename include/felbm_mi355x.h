@@ -193,6 +193,19 @@ int  felbm_mrt_matrices(double *M, double *Minv);
 /* last kernel timings of one step, in ms: moments, c-stencil, mu-stencil, collide */
 int  felbm_kernel_times(felbm_engine *e, int nsteps, float *ms4);
 
+/* Tile-shape autotuner for the batched engine (no reference counterpart: the
+ * reference launches fixed shapes).  Times each pass at 64, 128 and 256 sites
+ * per workgroup over nsteps steps (after two warm-up steps per shape) on the
+ * instances present, keeps the fastest shape per pass, and restores every
+ * instance's state bit for bit, so the tuning steps leave no trace.  Refused
+ * (FELBM_ESTATE) while tracers are present.  bs4: the chosen sites per
+ * workgroup of moments, c-stencil, mu-stencil, collide; ms12: pass times in ms
+ * as [shape 64, 128, 256][pass]; either may be NULL.  Results do not depend
+ * on the shapes. */
+int  felbm_autotune(felbm_engine *e, int nsteps, int *bs4, float *ms12);
+/* set the sites per workgroup of the four passes (64, 128 or 256 each) */
+int  felbm_set_block_sizes(felbm_engine *e, const int *bs4);
+
 /* streaming-bandwidth probe: `reps` device-to-device copies of `bytes`;
  * gbs = (read + write) bytes / s.  Also the calibration kernel for the HBM
  * byte counters.  No reference counterpart (measurement aid). */
