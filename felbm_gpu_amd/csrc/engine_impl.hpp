@@ -694,8 +694,10 @@ public:
         for (int w = 0; w < 2 && rc == FELBM_OK; ++w) rc = launch_step(false);
         if (rc == FELBM_OK) rc = kernel_times(nsteps, ms[bi]);
       }
+      // the state goes back whatever happened while timing
       for (auto &x : saved)
-        if (rc == FELBM_OK && hipMemcpyAsync(x.first->base, x.second, x.first->bytes, hipMemcpyDeviceToDevice, stream_) != hipSuccess) {
+        if (hipMemcpyAsync(x.first->base, x.second, x.first->bytes, hipMemcpyDeviceToDevice, stream_) != hipSuccess &&
+            rc == FELBM_OK) {
           set_error("state restore failed (autotune)"); rc = FELBM_EHIP;
         }
       if (hipStreamSynchronize(stream_) != hipSuccess && rc == FELBM_OK) { set_error("autotune synchronisation failed"); rc = FELBM_EHIP; }
