@@ -89,6 +89,7 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       float cN[Q], pN[Q], c2[Q], p2[Q], lN[Q];
       const Rec4<float> r0 = bld_rec4((const float *)nullptr, rF, io, F_REC*nsb);
       const float craw = bld_nt<float>(rF, io, F_CRAW*nsb);      // read once
+      const float clo = bld_nt<float>(rF, io, F_CLO*nsb);        // Sigma h - c_raw (moments pass)
       {
         // a neighbour that is not a site takes the own record's values
         unsigned vo2[Q];
@@ -111,7 +112,10 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       // the small terms of h, so the populations carry the correction, and
       // stream it, as the FP64 state does.  0 when no correction applies.
       const float phi_raw = craw*(1.0f - craw);        // as corrected() forms it
-      const float cres = (craw - c) - lam*(phi_raw > 0.0f ? phi_raw : 0.0f);
+      // with the moments pass's remainder: Sigma h of the next step starts from
+      // the exact sum of this step's populations, corrected, not from the FP32 c
+      // (an imposed open-boundary c carries no remainder)
+      const float cres = (craw - c) - lam*(phi_raw > 0.0f ? phi_raw : 0.0f) + (eb >= 0 ? 0.0f : clo);
       const float drho = d.drho;
       const float CS2 = 1.0f/3.0f;
       const float rho = ovp ? ldg(ovp, 1) : d.rho1 + drho*c;

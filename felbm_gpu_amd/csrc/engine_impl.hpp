@@ -156,7 +156,7 @@ public:
     real *f = (real *)(S.base + o_f);
     d.craw = f; d.p0 = f + F_PJ * ns; d.jx = d.jy = d.jz = d.p0;      // [p0, jx, jy, jz] per site
     d.c = d.ps = f + F_CP * ns;                                        // [c, p*] per site
-    d.mu = f + F_MU * ns; d.lapmu = f + F_LAPMU * ns;
+    d.mu = f + F_MU * ns; d.clo = f + F_CLO * ns;
     export_off_ = 9;   // exports at f + 9*ns .. 12*ns
     d.acc = (unsigned long long *)(S.base + o_a);
     set_params(S);

@@ -87,7 +87,7 @@ struct Inst {
   unsigned nwords;
   real *h, *g0, *g1;
   real *craw, *p0, *jx, *jy, *jz;
-  real *c, *mu, *ps, *lapmu;
+  real *c, *mu, *ps, *clo;      // clo: FP32 remainder Sigma h - c_raw of the moments pass
   unsigned long long *acc;
   real rho0, rho1, drho, kappa, four_beta, bnd_coeff, tau0, tau1, mobility, mrt_lambda;
   real gx, gy, gz, fx, fy, fz, ff;
@@ -226,13 +226,14 @@ __device__ __forceinline__ void bst(double v, rsrc_t r, unsigned vo, unsigned so
 { __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, (int)vo, (int)so, 0); }
 
 // field block layout (ns values each): c_raw; [p0, jx, jy, jz] interleaved per
-// site (4 ns); [c, p*] interleaved per site (2 ns); mu; lapmu; then 4 x ns
+// site (4 ns); [c, p*] interleaved per site (2 ns); mu; the FP32 remainder
+// Sigma h - c_raw of the moments pass (unused in FP64); then 4 x ns
 // holding either the per-site records [c, p*, lapmu, mu] the mu-stencil pass
 // writes for the collision (one vector load per neighbour), or, between steps,
 // the exported rho, ux, uy, uz of a field download.  A pass reads its own
 // site's inputs with one vector access per group: every vector memory
 // instruction costs the address unit the same whatever its width.
-enum { F_CRAW = 0, F_PJ = 1, F_CP = 5, F_MU = 7, F_LAPMU = 8, F_REC = 9, F_COUNT = 13 };
+enum { F_CRAW = 0, F_PJ = 1, F_CP = 5, F_MU = 7, F_CLO = 8, F_REC = 9, F_COUNT = 13 };
 
 // record loads/stores: (c, p*) or (c, p*, lapmu, mu) of one site, vo = site
 // byte offset in a scalar field (i * sizeof(real)); the record sits at 4*vo
@@ -650,18 +651,28 @@ __global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ 
     unsigned vo[Q], fl, so;
     ring1<real, OPEN>(d, rW, nb, io, vo, fl, so);
     (void)so;
-    real c = 0, p = 0, ux = 0, uy = 0, uz = 0;
+    // c accumulates in FP64 (the FP64 state sums alike).  The FP32 field c
+    // keeps Sigma h to half an ulp (6e-8 near c = 1), below which the slow
+    // changes of a bulk phase fall; the exact remainder goes to the collision
+    // pass, which adds it to the populations it writes, so Sigma h carries on
+    // from step to step without the rounding.
+    double cs = 0;
+    real p = 0, ux = 0, uy = 0, uz = 0;
     #pragma unroll
     for (int k = 0; k < Q; ++k) {           // streaming (gather) and moments
       const unsigned off = pull_off(k, vo, fl, io, nsb), soff = pull_soff(k, nsb);
       real hv = bld<real>(rH, off, soff), gv = bld<real>(rG, off, soff);
-      c += hv; p += gv;
+      cs += (double)hv; p += gv;
       acc_vec<real>(k, gv, ux, uy, uz);
     }
+    const real c = (real)cs;
+    if constexpr (sizeof(real) == 4)        // read once, by this site's own collision
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)(cs - (double)c)), rF, (int)io,
+                                            (int)(F_CLO*nsb), 2);
     // [p0, j] is read once, by this site's own c pass: written non-temporal
     bst(c, rF, io, F_CRAW*nsb); bst_pj<2>(p, ux, uy, uz, rF, io, F_PJ*nsb);
     if (d.mass_on) {              // k_mass_weight, device_engine.cuh:1109-1126
-      double cm = (double)c;
+      double cm = cs;
       double phi = cm*(1.0 - cm); if (phi < 0.0) phi = 0.0;
       fx_add(m, cm); fx_add(w, phi);
     }
@@ -1284,11 +1295,18 @@ __global__ void __launch_bounds__(256) k_mass_single(Inst<real> d, unsigned cur,
   if (i < d.n) {
     real hv[Q];
     gather_site(d, d.h, i, hv);
-    real c = 0;
+    double cs = 0;                  // summed as the moments pass sums it
     #pragma unroll
-    for (int k = 0; k < Q; ++k) c += hv[k];
-    if (apply) c = corrected<real>(c, true, lambda);
+    for (int k = 0; k < Q; ++k) cs += (double)hv[k];
+    const real c0 = (real)cs;
+    const real c = apply ? corrected<real>(c0, true, lambda) : c0;
     double cm = (double)c;
+    if constexpr (sizeof(real) == 4) {
+      // the populations carry the exact sum and the exact correction (the
+      // moments remainder and the correction residual, k_collide_f32)
+      const float phi = c0*(1.0f - c0);                 // as corrected() forms it
+      cm = cs - ((apply && phi > 0.0f) ? (double)(float)lambda*(double)phi : 0.0);
+    }
     double phi = cm*(1.0 - cm); if (phi < 0.0) phi = 0.0;
     fx_add(m, cm); fx_add(w, phi);
   }
@@ -1306,9 +1324,10 @@ __global__ void __launch_bounds__(256) k_inject_copy(Inst<real> d, int apply, do
   const int ns = d.ns;
   real hv[Q];
   gather_site(d, d.h, i, hv);
-  real c = 0;
+  double cs = 0;                    // summed as the moments pass sums it
   #pragma unroll
-  for (int k = 0; k < Q; ++k) c += hv[k];
+  for (int k = 0; k < Q; ++k) cs += (double)hv[k];
+  const real c = (real)cs;
   double cm = (double)c, phi = cm*(1.0 - cm);
   bool inj = apply && phi > 0.0;
   double delta_c = -lambda*phi;

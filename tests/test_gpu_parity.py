@@ -154,8 +154,10 @@ def _fp32_errors(o, g, steps):
     return dc, du
 
 
-# FP32 tolerance against the FP64 oracle: 1e-5 in c over 20 and over 200
-# steps, and 5e-8 in u (measured: 8.6e-7 and 1.0e-6 in c, 6e-9 and 8e-9 in u).
+# FP32 tolerance against the FP64 oracle: 1e-6 in c over 20 and over 200
+# steps, and 2e-8 in u (measured: 1.3e-7 and 1.6e-7 in c, 2.4e-9 and 2.8e-9 in
+# u; 1.0e-6 in c before the populations carried the moments pass's rounding
+# remainder, 3.1e-6 in c and 1.2e-7 in u with absolute FP32 pressures).
 # The u bound holds only with the pressure stored as a deviation from the rest
 # pressure (p - 1, g_k - w_k): with absolute FP32 pressures u was off by 1.2e-7
 # after 200 steps (and c by 3.1e-6).
@@ -165,10 +167,10 @@ def test_fp32_within_tolerance_of_fp64_oracle():
     o, g = make_pair(solid, prm, 1, 1, precision="f32")
     dc, du = _fp32_errors(o, g, 20)
     _say("fp32 mrt 20 steps max|dc|", dc, "max|du|", du)
-    assert dc <= 1e-5 and du <= 5e-8
+    assert dc <= 1e-6 and du <= 2e-8
     dc, du = _fp32_errors(o, g, 180)
     _say("fp32 mrt 200 steps max|dc|", dc, "max|du|", du)
-    assert du <= 5e-8 and dc <= 1e-5
+    assert du <= 2e-8 and dc <= 1e-6
 
 
 # Without the mass correction, the FP32 run's total c may differ from the FP64
@@ -179,7 +181,8 @@ def test_fp32_within_tolerance_of_fp64_oracle():
 # G_0 = w_0 (1 - 1.5 u^2) to FP32 before it multiplies c: at a given |u| the
 # rest population's offset from w_0 is the same fraction of an ulp at every
 # site, so every bulk site gained or lost the same mass (measured 1.3e-9 per
-# step); h_k = w_k c + (w_k u_term_k) c + G_k s_k, rounded once, gives 2.4e-11.
+# step); h_k = w_k c + (w_k u_term_k) c + G_k s_k, rounded once, gives 2.4e-11,
+# and carrying the moments pass's rounding remainder into the populations 6e-12.
 def test_fp32_mass_drift_below_weight_rounding():
     edge, steps = 64, 400
     sol = W.spheres(edge, edge, edge, 5.0, 9.0, 0.4, 1000)
@@ -194,16 +197,17 @@ def test_fp32_mass_drift_below_weight_rounding():
         e.free()
     rel = abs(gain["f32"][0] - gain["f64"][0]) / gain["f64"][1] / steps
     _say("fp32 - fp64 mass gain per step, relative: %.3e" % rel)
-    assert rel < 2e-10
+    assert rel < 5e-11
 
 
 # The mass correction at an obtuse contact angle: sweep instance 42 (96^3 at
 # porosity 0.46, 139 degrees, mobility coefficient 0.012).  Near the interface
 # the correction per step is below half an ulp of the FP32 c; unless what the
 # rounding drops is carried into the populations, the same rounding at the
-# same sites every step left c off by 2.1e-4 after 1000 steps (measured 1.1e-5
-# with it, p99 1.1e-6; FP64 runs from initial states 1e-7 apart stay within
-# 1.2e-7 of each other, so the dynamics do not amplify).
+# same sites every step left c off by 2.1e-4 after 1000 steps (1.1e-5 with it;
+# measured 6.6e-7, p99 1.7e-7, once the moments pass's rounding remainder is
+# carried too; FP64 runs from initial states 1e-7 apart stay within 1.2e-7 of
+# each other, so the dynamics do not amplify).
 def test_fp32_mass_correction_obtuse_angle_sweep_instance():
     from felbm_gpu_amd import serving
     spec = W.sweep_instances(48, 1000)[42]
@@ -220,7 +224,7 @@ def test_fp32_mass_correction_obtuse_angle_sweep_instance():
     dc = np.abs(f["f32"]["c"] - f["f64"]["c"])
     du = max(np.abs(f["f32"][k] - f["f64"][k]).max() for k in ("ux", "uy", "uz"))
     _say("fp32 sweep instance 42, 1000 steps: max|dc| %.3e p99 %.3e max|du| %.3e" % (dc.max(), np.percentile(dc, 99), du))
-    assert dc.max() <= 3e-5 and np.percentile(dc, 99) <= 5e-6 and du <= 1e-6
+    assert dc.max() <= 5e-6 and np.percentile(dc, 99) <= 1e-6 and du <= 1e-7
 
 
 def test_batch_equals_alone_and_graph_equals_eager():
@@ -286,10 +290,10 @@ def test_fp32_bgk_within_tolerance():
     o, g = make_pair(w["solid"], w["params"], 0, 1, precision="f32", c0=w["c0"])
     dc, du = _fp32_errors(o, g, 20)
     _say("fp32 bgk 20 steps max|dc|", dc, "max|du|", du)
-    assert dc <= 1e-5 and du <= 1e-7
+    assert dc <= 2e-6 and du <= 1e-7
     dc, du = _fp32_errors(o, g, 180)
     _say("fp32 bgk 200 steps max|dc|", dc, "max|du|", du)
-    assert du <= 1e-7 and dc <= 2e-5
+    assert du <= 1e-7 and dc <= 3e-6
 
 
 @pytest.mark.parametrize("use_mrt", [0, 1])
