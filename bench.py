@@ -193,7 +193,7 @@ def main():
     if args.serve > 0:
         eng.free()
         from felbm_gpu_amd import serving
-        res = serving.run_serve(args.serve, rate=4.0, seed=2024, precision="f32", log=log)
+        res = serving.run_serve(args.serve, rate=4.0, seed=2024, precision="f32", log=log, check_fp64_steps=2000)
         served = {k: v for k, v in res.items() if k != "done"}
         served["requests_steps"] = [d["steps"] for d in res["done"]]
         log("serve: %s" % json.dumps(served))

@@ -280,8 +280,23 @@ class Server:
                 "lattice_updates": lu, "done": done}
 
 
+def _served_fields(problems, precision, steps):
+    """Fields of every request problem after `steps` steps, all in one batch."""
+    be = GpuBackend(precision)
+    try:
+        hs = [be.admit(pr)[0] for pr in problems]
+        be.batch.step(steps)
+        be.batch.synchronize()
+        return [h.download() for h in hs]
+    finally:
+        be.close()
+
+
 def run_serve(count, rate=4.0, seed=2024, precision="f32", check_every=500, tol=1e-6, max_steps=20000,
-              hbm_budget=200e9, log=None):
+              hbm_budget=200e9, log=None, check_fp64_steps=0):
+    """The served stream; check_fp64_steps > 0 also runs every request problem
+    that many steps in FP32 and in FP64 and reports the largest FP32 - FP64
+    difference of c and u over them."""
     reqs = make_requests(count, rate, seed)
     t_gen = time.time()
     problems = {r["id"]: request_problem(r) for r in reqs}
@@ -293,4 +308,18 @@ def run_serve(count, rate=4.0, seed=2024, precision="f32", check_every=500, tol=
     finally:
         be.close()
     res["rate_per_s"] = rate
+    if check_fp64_steps > 0 and precision == "f32":
+        probs = [problems[r["id"]] for r in reqs]
+        f32 = _served_fields(probs, "f32", check_fp64_steps)
+        f64 = _served_fields(probs, "f64", check_fp64_steps)
+        finite = lambda f: all(np.isfinite(f[k]).all() for k in f)
+        pairs = [(a, b) for a, b in zip(f32, f64) if finite(b)]
+        dc = [float(np.abs(a["c"] - b["c"]).max()) for a, b in pairs]
+        du = [max(float(np.abs(a[k] - b[k]).max()) for k in ("ux", "uy", "uz")) for a, b in pairs]
+        res["fp32_vs_fp64"] = {"steps": int(check_fp64_steps), "compared": len(pairs),
+                               "fp64_nonfinite": len(probs) - len(pairs),
+                               "max_dc": max(dc) if dc else None, "max_du": max(du) if du else None,
+                               "median_request_dc": float(np.median(dc)) if dc else None}
+        if log:
+            log("serve: fp32 vs fp64 %s" % res["fp32_vs_fp64"])
     return res
