@@ -80,3 +80,38 @@ def test_autotune_leaves_state_and_steps_unchanged():
             for g in i2:
                 g.free()
             b2.free()
+
+
+def _open_batch(precision):
+    from felbm_gpu_amd import serving
+    reqs = serving.make_requests(2, rate=4.0, seed=91, edge_range=(24, 30))
+    be = serving.GpuBackend(precision)
+    hs = [be.admit(serving.request_problem(r))[0] for r in reqs]
+    prm = W.derive_params(theta_deg=60.0, acceleration=(1e-5, 0, 0))
+    s = W.spheres(18, 18, 18, 3.0, 5.0, 0.55, 31)
+    g = MultiPhaseGPU(batch=be.batch).init(s, prm, {"use_mrt": 1, "correct_op_mass": 1})
+    g.init_rest(W.slab_c(s, 0, 0, 9, 4.0), 1.0)
+    g.record_target_mass()
+    return be, hs + [g]
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_shapes_identical_with_open_boundaries(precision):
+    """Open-boundary and periodic instances in one batch, FP32 and FP64: every
+    shape gives the same state."""
+    ref = None
+    for shapes in (None, [64, 128, 64, 128], [128, 64, 256, 64]):
+        be, insts = _open_batch(precision)
+        if shapes is not None:
+            be.batch.set_block_sizes(shapes)
+        be.batch.step(15)
+        got = [g.download_state() for g in insts]
+        for g in insts:
+            g.free()
+        be.close()
+        if ref is None:
+            ref = got
+            continue
+        _say(precision, "shapes", shapes)
+        for (h0, g0), (h1, g1) in zip(ref, got):
+            assert np.array_equal(h0, h1) and np.array_equal(g0, g1), shapes
