@@ -114,7 +114,8 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       const float phi_raw = craw*(1.0f - craw);        // as corrected() forms it
       // with the moments pass's remainder: Sigma h of the next step starts from
       // the exact sum of this step's populations, corrected, not from the FP32 c
-      // (an imposed open-boundary c carries no remainder)
+      // (open-boundary nodes carry none: their c_raw may hold an imposed value
+      // or a mirrored vertex's, which the remainder does not belong to)
       const float cres = (craw - c) - lam*(phi_raw > 0.0f ? phi_raw : 0.0f) + (eb >= 0 ? 0.0f : clo);
       const float drho = d.drho;
       const float CS2 = 1.0f/3.0f;
