@@ -52,7 +52,7 @@ struct Slot {
   // tracers
   int tr_np = 0, tr_substeps = 1;
   unsigned long long tr_seed = 0;
-  char *tr_base = nullptr;        // pos [3np] f64, dm [np] f64, uvw [3np] f64, blocked [np] i32, tu [3ns] real
+  char *tr_base = nullptr;        // pos [3np] f64, dm [np] f64, uvw [3np] f64, blocked [np] i32, tu [ns][4] real
   double *tr_pos = nullptr, *tr_dm = nullptr, *tr_uvw = nullptr;
   int *tr_blocked = nullptr;
   // Lagrangian stretching (rho_hat, log_rho, block partials, row ring)
@@ -512,7 +512,7 @@ public:
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off = align_up(off + b, 256); return o; };
     const size_t o_p = take(3 * n * 8), o_d = take(n * 8), o_u = take(3 * n * 8), o_b = take(n * 4);
-    const size_t o_t = take(4 * ns * sizeof(real));   // u (3) and c for the phase split
+    const size_t o_t = take(4 * ns * sizeof(real));   // per site: u (3) and c for the phase split
     if (hipMalloc((void **)&S.tr_base, off) != hipSuccess) { set_error("device allocation failed"); return FELBM_ENOMEM; }
     FM_HIP(hipMemsetAsync(S.tr_base, 0, off, stream_));
     S.tr_pos = (double *)(S.tr_base + o_p); S.tr_dm = (double *)(S.tr_base + o_d);

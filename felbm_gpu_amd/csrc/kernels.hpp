@@ -103,7 +103,7 @@ struct Inst {
   int trk;                     // tracers present
   int tr_vskip;                // refresh the snapshot every tr_vskip steps
   int tr_phase;                // the snapshot also holds c (phase split of the stretching rows)
-  real *tu;                    // [3 or 4][ns] snapshot of u (the field pass's u*), then c
+  real *tu;                    // [ns][4] snapshot of (u*, c) of the field pass, one record per site
 };
 
 // site classes of the open boundary
@@ -933,8 +933,10 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
     const unsigned long long ts = ldg(d.acc, ACC_FLAGS + 2);
     if (ts % (unsigned long long)d.tr_vskip == 0) {
       const long ns = d.ns;
-      stg(d.tu, i, ux); stg(d.tu, ns + i, uy); stg(d.tu, 2*ns + i, uz);
-      if (d.tr_phase) stg(d.tu, 3*ns + i, c);
+      // one record per site, [ux, uy, uz, c], in one vector store (c is read
+      // only for the phase split of the stretching rows)
+      typedef real v4 __attribute__((ext_vector_type(4)));
+      ((__attribute__((address_space(1))) v4 *)d.tu)[i] = v4{ux, uy, uz, c};
     }
   }
 }

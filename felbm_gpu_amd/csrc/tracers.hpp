@@ -83,7 +83,6 @@ __device__ inline void interp_u(const Inst<real> &d, const double (&p)[3], doubl
   const double fl0 = floor(p[0]), fl1 = floor(p[1]), fl2 = floor(p[2]);
   const int x0 = (int)fl0, y0 = (int)fl1, z0 = (int)fl2;
   const double fx = p[0] - fl0, fy = p[1] - fl1, fz = p[2] - fl2;
-  const long ns = d.ns;
   for (int cz = 0; cz < 2; ++cz)
     for (int cy = 0; cy < 2; ++cy)
       for (int cx = 0; cx < 2; ++cx) {
@@ -92,9 +91,9 @@ __device__ inline void interp_u(const Inst<real> &d, const double (&p)[3], doubl
         const int s = voxel_site(d, x, y, z);
         if (s < 0) continue;
         const double w = ((cx ? fx : 1.0 - fx)*(cy ? fy : 1.0 - fy))*(cz ? fz : 1.0 - fz);
-        u[0] += w*(double)ldg(d.tu, s);
-        u[1] += w*(double)ldg(d.tu, ns + s);
-        u[2] += w*(double)ldg(d.tu, 2*ns + s);
+        u[0] += w*(double)ldg(d.tu, 4L*s);
+        u[1] += w*(double)ldg(d.tu, 4L*s + 1);
+        u[2] += w*(double)ldg(d.tu, 4L*s + 2);
       }
 }
 
@@ -108,7 +107,6 @@ __device__ inline void interp_uJ(const Inst<real> &d, const double (&p)[3], doub
   const double fl0 = floor(p[0]), fl1 = floor(p[1]), fl2 = floor(p[2]);
   const int x0 = (int)fl0, y0 = (int)fl1, z0 = (int)fl2;
   const double fx = p[0] - fl0, fy = p[1] - fl1, fz = p[2] - fl2;
-  const long ns = d.ns;
   for (int cz = 0; cz < 2; ++cz)
     for (int cy = 0; cy < 2; ++cy)
       for (int cx = 0; cx < 2; ++cx) {
@@ -119,7 +117,7 @@ __device__ inline void interp_uJ(const Inst<real> &d, const double (&p)[3], doub
         const double wx = cx ? fx : 1.0 - fx, wy = cy ? fy : 1.0 - fy, wz = cz ? fz : 1.0 - fz;
         const double w = (wx*wy)*wz;
         const double gx = (cx ? 1.0 : -1.0)*(wy*wz), gy = (cy ? 1.0 : -1.0)*(wx*wz), gz = (cz ? 1.0 : -1.0)*(wx*wy);
-        const double v[3] = { (double)ldg(d.tu, s), (double)ldg(d.tu, ns + s), (double)ldg(d.tu, 2*ns + s) };
+        const double v[3] = { (double)ldg(d.tu, 4L*s), (double)ldg(d.tu, 4L*s + 1), (double)ldg(d.tu, 4L*s + 2) };
         for (int a = 0; a < 3; ++a) {
           u[a] += w*v[a];
           J[a][0] += gx*v[a]; J[a][1] += gy*v[a]; J[a][2] += gz*v[a];
@@ -141,7 +139,7 @@ __device__ inline double nearest_c(const Inst<real> &d, const double (&p)[3])
     if (!wrap_voxel(v[a], N[a], P[a])) return __builtin_nan("");
   }
   const int s = voxel_site(d, v[0], v[1], v[2]);
-  return s < 0 ? __builtin_nan("") : (double)ldg(d.tu, 3L*d.ns + s);
+  return s < 0 ? __builtin_nan("") : (double)ldg(d.tu, 4L*s + 3);
 }
 
 // per-step stretching state of one slot's tracers
