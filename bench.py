@@ -163,18 +163,22 @@ def main():
     f32_err = None
     if args.err_steps > 0 and w["precision"] == "f32":
         eng.free()
-        fields = {}
+        fields, rates = {}, {}
         for prec in ("f64", "f32"):
             e2 = MultiPhaseGPU(precision=prec).init(w["solid"], w["params"], w["switches"])
             e2.init_rest(w["c0"], w["p0"])
             e2.record_target_mass()
+            e2.batch.synchronize()
+            t_e = time.perf_counter()
             for _ in range(args.err_steps // 1000):     # progress between chunks
                 e2.step(1000)
                 e2.batch.synchronize()
             e2.step(args.err_steps % 1000)
+            e2.batch.synchronize()
+            rates[prec] = e2.n * args.err_steps / (time.perf_counter() - t_e) / 1e6
             fields[prec] = e2.download()
             e2.free()
-        f32_err = {"steps": args.err_steps}
+        f32_err = {"steps": args.err_steps, "MLUPS_over_run": rates}
         for k in ("rho", "c", "ux", "uy", "uz", "p"):
             f32_err[k] = float(np.abs(fields["f32"][k] - fields["f64"][k]).max())
         for k in ("c", "ux"):
