@@ -141,9 +141,18 @@ public:
     size_t o_m = take(sizeof(int2) * (size_t)(align_up((size_t)S.nblocks, 8)));
     S.bytes = off;
     if (hipMalloc((void **)&S.base, off) != hipSuccess) { set_error("device allocation failed"); return FELBM_ENOMEM; }
-    FM_HIP(hipMemsetAsync(S.base, 0, off, stream_));
-    FM_HIP(hipMemcpyAsync(S.base + o_w, words.data(), words.size() * 8, hipMemcpyHostToDevice, stream_));
-    FM_HIP(hipMemcpyAsync(S.base + o_c, coords.data(), coords.size() * 4, hipMemcpyHostToDevice, stream_));
+    {
+      hipError_t he = hipMemsetAsync(S.base, 0, off, stream_);
+      if (he == hipSuccess) he = hipMemcpyAsync(S.base + o_w, words.data(), words.size() * 8, hipMemcpyHostToDevice, stream_);
+      if (he == hipSuccess) he = hipMemcpyAsync(S.base + o_c, coords.data(), coords.size() * 4, hipMemcpyHostToDevice, stream_);
+      if (he == hipSuccess) he = hipStreamSynchronize(stream_);
+      if (he != hipSuccess) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipFree(S.base);
+        set_error(std::string("add_instance: ") + hipGetErrorName(he));
+        return FELBM_EHIP;
+      }
+    }
     Inst<real> &d = S.d;
     d.n = S.n; d.ns = S.ns; d.Nx = S.Nx; d.Ny = S.Ny; d.Nz = S.Nz; d.wpr = wpr;
     d.sx = bx; d.sxy = bx + by; d.mx = (1u << bx) - 1u; d.my = (1u << by) - 1u;
@@ -171,7 +180,15 @@ public:
     S.used = true;
     slots_[sid] = S;
     dirty_ = true;
-    if (int e = rebuild()) return e;
+    if (int e = rebuild()) {
+      // roll the slot back: the engine keeps the instances it had
+      (void)hipStreamSynchronize(stream_);
+      (void)hipFree(slots_[sid].base);
+      slots_[sid] = Slot<real>();
+      dirty_ = true;
+      (void)rebuild();
+      return e;
+    }
     *slot = sid;
     return FELBM_OK;
   }
@@ -179,6 +196,7 @@ public:
   int remove_instance(int slot) override {
     if (int e = check(slot)) return e;
     FM_HIP(hipStreamSynchronize(stream_));
+    drop_graph();
     FM_HIP(hipFree(slots_[slot].base));
     if (slots_[slot].ob_base) FM_HIP(hipFree(slots_[slot].ob_base));
     if (slots_[slot].tr_base) FM_HIP(hipFree(slots_[slot].tr_base));
@@ -444,6 +462,7 @@ public:
       }
     }
     FM_HIP(hipStreamSynchronize(stream_));
+    drop_graph();                     // a captured step graph holds the boundary lists
     if (S.ob_base) { FM_HIP(hipFree(S.ob_base)); S.ob_base = nullptr; }
     const size_t ent_bytes = align_up((size_t)n * sizeof(int), 256);
     const size_t bytes = ent_bytes + std::max<size_t>(1, E.size()) * 8 * sizeof(real);
@@ -503,6 +522,7 @@ public:
     }
     Slot<real> &S = slots_[slot];
     FM_HIP(hipStreamSynchronize(stream_));
+    drop_graph();                     // a captured step graph holds the tracer buffers
     if (S.tr_base) { FM_HIP(hipFree(S.tr_base)); S.tr_base = nullptr; }
     if (S.tr_sbase) { FM_HIP(hipFree(S.tr_sbase)); S.tr_sbase = nullptr; }
     S.tr_np = 0; S.d.trk = 0; S.d.tu = nullptr; S.d.tr_phase = 0; S.st = Stretch{}; S.tr_drained = 0;
@@ -553,6 +573,7 @@ public:
     Slot<real> &S = slots_[slot];
     if (!S.tr_np) { set_error("no tracers on this slot"); return FELBM_ESTATE; }
     FM_HIP(hipStreamSynchronize(stream_));
+    drop_graph();                     // a captured step graph holds the stretching buffers
     if (S.tr_sbase) { FM_HIP(hipFree(S.tr_sbase)); S.tr_sbase = nullptr; }
     S.st = Stretch{}; S.d.tr_phase = 0; dirty_desc_ = true;
     if (!enable) return FELBM_OK;
@@ -756,6 +777,14 @@ private:
   bool stats_launched_ = false;
   hipGraphExec_t graph_exec_ = nullptr;
   int graph_steps_ = 0; unsigned graph_ver_ = ~0u, graph_T_ = 0;
+
+  // A captured step graph holds its kernels' arguments by value: the tracer
+  // buffers and counts, the block maps and shapes.  Dropped before any of them
+  // is freed or changed; the next step_graph captures again.
+  void drop_graph() {
+    if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+    ++map_ver_;
+  }
 
   int check(int slot) {
     if (slot < 0 || slot >= (int)slots_.size() || !slots_[slot].used) { set_error("invalid slot"); return FELBM_EINVAL; }

@@ -115,9 +115,9 @@ def test_stretching_matches_restatement_fp64():
 
 
 def test_tracers_in_graph_replays_equal_eager():
-    """The tracers run on their own stream beside the mu pass and join before
-    the collision pass; captured into a graph and replayed, they move exactly
-    as in eager steps (positions, stretching state and rows)."""
+    """The tracer kernels run on the engine stream between the mu pass and the
+    collision pass; captured into a graph and replayed, they move exactly as
+    in eager steps (positions, stretching state and rows)."""
     runs = []
     for graph in (False, True):
         o, g, t, dm = setup(vskip=2)
@@ -131,5 +131,30 @@ def test_tracers_in_graph_replays_equal_eager():
         rows, _ = g.tracers_stretch_rows()
         runs.append((xyz, blk, rh, lr, rows))
     _say("graph vs eager tracers: max|dx| %.3e" % float(np.abs(runs[0][0] - runs[1][0]).max()))
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+
+
+def test_graph_recaptured_after_tracers_reallocate():
+    """A step graph captured before tracers_init / tracers_stretching holds
+    the old tracer buffers as kernel arguments; the engine drops it when they
+    are reallocated.  step_graph, tracers_init (new tracers), stretching on,
+    step_graph again: positions, stretching state and fields equal the same
+    sequence run with eager steps."""
+    runs = []
+    for graph in (False, True):
+        o, g, t, dm = setup(vskip=1)
+        adv = (lambda k: g.batch.step_graph(4, k)) if graph else (lambda k: g.step(4 * k))
+        adv(1)
+        pos2, _ = W.seed_planes(g.solid, 150, (8.5,), axis=0, seed=9)
+        g.tracers_init(pos2, np.full(len(pos2), 1e-3), seed=5, substeps=1, velocity_skip=1)
+        adv(2)
+        g.tracers_stretching(True, phase_split=True, capacity=32)
+        adv(1)
+        xyz, _, blk = g.tracers_download()
+        rh, lr = g.tracers_stretch_state()
+        f = g.download()
+        runs.append((xyz, blk, rh, lr, f["c"], f["ux"]))
+    _say("graph after tracer reallocation vs eager: max|dx| %.3e" % float(np.abs(runs[0][0] - runs[1][0]).max()))
     for a, b in zip(runs[0], runs[1]):
         assert np.array_equal(a, b)
