@@ -87,6 +87,7 @@ SIGNATURES = {
     "felbm_download_fields": (_I, [_P, _I, _PD]),
     "felbm_set_open_bnd": (_I, [_P, _I, C.POINTER(OpenBnd), _PI32, _I, _PI32, _I, _PD, _PI32, _PI32, _I, _I]),
     "felbm_export_tables": (_I, [_P, _I] + [_PI32] * 10),
+    "felbm_export_tables_noalloc": (_I, [_P, _I] + [_PI32] * 10),
     "felbm_mrt_matrices": (_I, [_PD, _PD]),
     "felbm_kernel_times": (_I, [_P, _I, C.POINTER(C.c_float)]),
     "felbm_autotune": (_I, [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_float)]),

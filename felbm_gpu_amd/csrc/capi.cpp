@@ -2,6 +2,7 @@
 #include <cmath>
 #include <hip/hip_runtime.h>
 #include <string>
+#include <mutex>
 #include <new>
 #include "engine.hpp"
 #include "kernels.hpp"
@@ -9,6 +10,40 @@
 namespace fm {
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
+
+// Static device workspace of felbm_export_tables_noalloc: the staging area its
+// kernel writes the ten tables into before they are copied to the caller's
+// host arrays.  Every GPU the library runs on holds its own copy, at its own
+// address, until the process ends.  In host code `scratch` names a host-side
+// stand-in, never the device memory: its address on each device comes from
+// hipGetSymbolAddress, once per device, at engine creation (scratch_init).
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+int scratch_init(int device)
+{
+  if (device < 0 || device >= kMaxScratchDevices) { set_error("device index beyond the workspace table"); return FELBM_EUNSUPPORTED; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return FELBM_OK;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != device) { set_error("workspace set-up: device not current"); return FELBM_EHIP; }
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_error(std::string("workspace set-up: hipGetSymbolAddress failed: ") + hipGetErrorName(e));
+    return FELBM_EHIP;
+  }
+  g_scratch_ptr[device] = p;
+  return FELBM_OK;
+}
+
+void *scratch_ptr(int device)
+{
+  if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 }
 
 struct felbm_engine { fm::EngineBase *impl; int precision; };
@@ -110,6 +145,13 @@ int felbm_export_tables(felbm_engine *e, int slot, int32_t *cdm, int32_t *cdp, i
   ENG(e);
   int32_t *t[10] = { cdm, cdp, bcase, bc1, bc2, avgnext, lcase, lc1, lc2, src };
   return e->impl->export_tables(slot, t);
+}
+int felbm_export_tables_noalloc(felbm_engine *e, int slot, int32_t *cdm, int32_t *cdp, int32_t *bcase, int32_t *bc1,
+                                int32_t *bc2, int32_t *avgnext, int32_t *lcase, int32_t *lc1, int32_t *lc2, int32_t *src)
+{
+  ENG(e);
+  int32_t *t[10] = { cdm, cdp, bcase, bc1, bc2, avgnext, lcase, lc1, lc2, src };
+  return e->impl->export_tables_noalloc(slot, t);
 }
 int felbm_kernel_times(felbm_engine *e, int nsteps, float *ms4) { ENG(e); if (!ms4) return FELBM_EINVAL; return e->impl->kernel_times(nsteps, ms4); }
 int felbm_autotune(felbm_engine *e, int nsteps, int *bs4, float *ms12) { ENG(e); return e->impl->autotune(nsteps, bs4, ms12); }

@@ -1,5 +1,6 @@
 // engine.hpp -- precision-independent engine interface behind the C ABI.
 #pragma once
+#include <cstddef>
 #include <cstdint>
 #include <string>
 #include "felbm_mi355x.h"
@@ -7,6 +8,14 @@
 namespace fm {
 
 void set_error(const std::string &msg);
+
+// Static device workspace (capi.cpp) of the allocation-free table export:
+// 32 MiB per GPU; scratch_init looks its address up once per device (the
+// device current), scratch_ptr returns it (nullptr before set-up).
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr int kMaxScratchDevices = 64;
+int scratch_init(int device);
+void *scratch_ptr(int device);
 
 struct EngineBase {
   virtual ~EngineBase() {}
@@ -34,6 +43,7 @@ struct EngineBase {
                            const int32_t *outlet, int n_outlet, const double *icoord,
                            const int32_t *bv, const int32_t *br, int n_buf, int copy_to_buffers) = 0;
   virtual int export_tables(int slot, int32_t **tabs) = 0;
+  virtual int export_tables_noalloc(int slot, int32_t **tabs) = 0;
   virtual int kernel_times(int nsteps, float *ms4) = 0;
   virtual int autotune(int nsteps, int *bs4, float *ms12) = 0;
   virtual int set_block_sizes(const int *bs4) = 0;

@@ -72,6 +72,8 @@ public:
     if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) { set_error("no HIP device available"); *err = FELBM_EHIP; return *err; }
     if (dev_ < 0 || dev_ >= nd) { set_error("device index out of range"); *err = FELBM_EINVAL; return *err; }
     if (hipSetDevice(dev_) != hipSuccess) { set_error("hipSetDevice failed"); *err = FELBM_EHIP; return *err; }
+    // the device address of the static workspace, once per device
+    if (int e = scratch_init(dev_)) { *err = e; return e; }
     if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) { set_error("stream create failed"); *err = FELBM_EHIP; return *err; }
     for (int q = 0; q < 10; ++q) if (hipEventCreate(&ev_[q]) != hipSuccess) { set_error("event create failed"); *err = FELBM_EHIP; return *err; }
     if (hipHostMalloc((void **)&pinned_stats_, sizeof(unsigned long long)*12*NSLOT*kMaxAsyncSlots, hipHostMallocDefault) != hipSuccess) { *err = FELBM_ENOMEM; return *err; }
@@ -510,6 +512,35 @@ public:
     if (le != hipSuccess) { set_error(std::string("export_tables: ") + hipGetErrorName(le)); return FELBM_EHIP; }
     o = 0;
     for (int q = 0; q < 10; ++q) { if (tabs[q]) std::memcpy(tabs[q], h.data() + o, sz[q] * 4); o += sz[q]; }
+    return FELBM_OK;
+  }
+
+  // export_tables with the staging area in the library's static workspace:
+  // no device allocation or release.  The kernel writes every entry of the ten
+  // tables, so nothing in the workspace needs clearing; calls that overlap on
+  // one GPU (two engines on two host threads) share the workspace and race.
+  int export_tables_noalloc(int slot, int32_t **tabs) override {
+    if (int e = check(slot)) return e;
+    Slot<real> &S = slots_[slot];
+    const size_t n = (size_t)S.n, Q1n = 18 * n, Vn = 19 * n;
+    const size_t sz[10] = { Q1n, Q1n, Q1n, Q1n, Q1n, Vn, Q1n, Q1n, Q1n, Vn };
+    size_t tot = 0; for (int q = 0; q < 10; ++q) tot += sz[q];
+    if (tot * sizeof(int) > kScratchBytes) {
+      set_error("export_tables_noalloc: " + std::to_string(S.n) + " sites need " + std::to_string(tot * sizeof(int)) +
+                " bytes of workspace; the static workspace holds " + std::to_string(kScratchBytes) + " (at most " +
+                std::to_string(kScratchBytes / (182 * sizeof(int))) + " sites)");
+      return FELBM_EINVAL;
+    }
+    int *buf = (int *)scratch_ptr(dev_);
+    if (!buf) { set_error("export_tables_noalloc: workspace not set up on this device"); return FELBM_ESTATE; }
+    int *p[10]; size_t o = 0;
+    for (int q = 0; q < 10; ++q) { p[q] = buf + o; o += sz[q]; }
+    hipLaunchKernelGGL(k_export_tables<real>, dim3((S.n + 255) / 256), dim3(256), 0, stream_, S.d,
+                       p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9]);
+    FM_LAUNCH();
+    for (int q = 0; q < 10; ++q)
+      if (tabs[q]) FM_HIP(hipMemcpyAsync(tabs[q], p[q], sz[q] * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
     return FELBM_OK;
   }
 

@@ -369,13 +369,16 @@ class MultiPhaseGPU:
         f = out.reshape(6, self.n)
         return {"rho": f[0], "c": f[1], "ux": f[2], "uy": f[3], "uz": f[4], "p": f[5]}
 
-    def export_tables(self):
+    def export_tables(self, noalloc=False):
+        """Reference-format tables; noalloc stages them in the library's static
+        device workspace (felbm_export_tables_noalloc) instead of a device
+        allocation."""
         n = self.n
         names = ("cdm", "cdp", "bcase", "bc1", "bc2", "avgnext", "lcase", "lc1", "lc2", "src")
         sizes = (18, 18, 18, 18, 18, 19, 18, 18, 18, 19)
         arrs = [np.empty(s * n, dtype=np.int32) for s in sizes]
-        check(self.lib.felbm_export_tables(self.h, self.slot,
-                                           *[a.ctypes.data_as(C.POINTER(C.c_int32)) for a in arrs]))
+        fn = self.lib.felbm_export_tables_noalloc if noalloc else self.lib.felbm_export_tables
+        check(fn(self.h, self.slot, *[a.ctypes.data_as(C.POINTER(C.c_int32)) for a in arrs]))
         return dict(zip(names, arrs))
 
     def free(self):

@@ -188,6 +188,17 @@ int  felbm_tracers_stretch_state(felbm_engine *e, int slot, double *rho_hat, dou
 int  felbm_export_tables(felbm_engine *e, int slot, int32_t *cdm, int32_t *cdp,
                          int32_t *bcase, int32_t *bc1, int32_t *bc2, int32_t *avgnext,
                          int32_t *lcase, int32_t *lc1, int32_t *lc2, int32_t *src);
+/* felbm_export_tables without any device allocation (no counterpart in the
+ * reference).  For callers that cannot allocate device memory: code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc.  The tables
+ * are staged in a static 32 MiB device workspace that every GPU the library
+ * runs on holds until the process ends, so instances of at most 46,091 sites
+ * (182 int32 per site) are accepted; larger ones return FELBM_EINVAL with a
+ * message.  The workspace is shared by every engine on a GPU: calls that
+ * overlap on one GPU (engines driven from two host threads) race. */
+int  felbm_export_tables_noalloc(felbm_engine *e, int slot, int32_t *cdm, int32_t *cdp,
+                                 int32_t *bcase, int32_t *bc1, int32_t *bc2, int32_t *avgnext,
+                                 int32_t *lcase, int32_t *lc1, int32_t *lc2, int32_t *src);
 /* the 19x19 MRT moment matrix and its inverse used by the engine */
 int  felbm_mrt_matrices(double *M, double *Minv);
 /* last kernel timings of one step, in ms: moments, c-stencil, mu-stencil, collide */
