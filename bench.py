@@ -78,6 +78,8 @@ def main():
                     help="keep 256 sites per workgroup in every pass instead of the autotuned shapes")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="only run the device-to-device copy probe of this size (counter calibration)")
+    ap.add_argument("--probe-stream-mb", type=int, default=0,
+                    help="only run the hand-written streaming probes (read, write, copy) of this size")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; runs across several GPUs are not supported "
@@ -94,8 +96,19 @@ def main():
         _lib.check(_lib.load().felbm_copy_probe(0, int(mb) << 20, int(reps), C.byref(gbs)))
         return gbs.value
 
+    def stream_probe(mb, reps):
+        out = {}
+        for mode, name in ((0, "read"), (1, "write"), (2, "copy")):
+            gbs = C.c_double()
+            _lib.check(_lib.load().felbm_stream_probe(0, int(mb) << 20, int(reps), mode, C.byref(gbs)))
+            out[name] = gbs.value
+        return out
+
     if args.probe_copy_mb:
         print(json.dumps({"copy_probe_mb": args.probe_copy_mb, "GBs": copy_probe(args.probe_copy_mb, 5)}))
+        return
+    if args.probe_stream_mb:
+        print(json.dumps({"stream_probe_mb": args.probe_stream_mb, "GBs": stream_probe(args.probe_stream_mb, 5)}))
         return
 
     t0 = time.time()
@@ -128,11 +141,12 @@ def main():
         kern[k] = {"ms": t, "alg_GBs": alg / (t * 1e-3) / 1e9 if t > 0 else None}
     coll = kern["collide"]
     copy_gbs = copy_probe(1024, 10)
+    stream_gbs = stream_probe(1024, 10)
     roof = {"bound": "hbm", "achieved": coll["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": coll["alg_GBs"] / HBM_PEAK_GBS, "traffic": _profile_traffic(args.workload),
             "kernel": "k_collide (fused MRT collision, gather streaming)",
             "alg_bytes_per_launch": ALG_VALUES["collide"] * bytes_per_val * n,
-            "copy_probe_GBs": copy_gbs}
+            "copy_probe_GBs": copy_gbs, "stream_probe_GBs": stream_gbs}
     mass_drift = abs(eng.current_mass() - eng.get_target_mass())
 
     # the same lattice carrying 12,000 tracers on the two slab interfaces, six

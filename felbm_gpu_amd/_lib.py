@@ -93,6 +93,7 @@ SIGNATURES = {
     "felbm_autotune": (_I, [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "felbm_set_block_sizes": (_I, [_P, C.POINTER(C.c_int)]),
     "felbm_copy_probe": (_I, [_I, C.c_int64, _I, _PD]),
+    "felbm_stream_probe": (_I, [_I, C.c_int64, _I, _I, _PD]),
     "felbm_last_error": (C.c_char_p, []),
     "felbm_version": (C.c_char_p, []),
     "felbm_geom_cylinders": (_I, [_I, _I, _I, _I, _D, _D, _D, C.c_uint64, _I, C.POINTER(C.c_uint8)]),

@@ -166,6 +166,84 @@ int felbm_mrt_matrices(double *M, double *Minv)
 
 }  // extern "C"
 
+namespace fm {
+// Streaming kernels of felbm_stream_probe: 16 bytes per lane, four independent
+// accesses in flight per lane per iteration, grid-stride over n4 f32x4 values,
+// non-temporal loads and stores (every byte is touched once per launch).
+__global__ void __launch_bounds__(256) k_stream_read(const f32x4 *__restrict__ a, size_t n4, float *__restrict__ sink)
+{
+  const size_t step = (size_t)gridDim.x * 256u;
+  size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+  float s = 0.0f;
+  for (; i + 3 * step < n4; i += 4 * step) {
+    const f32x4 v0 = __builtin_nontemporal_load(a + i), v1 = __builtin_nontemporal_load(a + i + step);
+    const f32x4 v2 = __builtin_nontemporal_load(a + i + 2 * step), v3 = __builtin_nontemporal_load(a + i + 3 * step);
+    s += (v0[0] + v1[1]) + (v2[2] + v3[3]);
+  }
+  for (; i < n4; i += step) { const f32x4 v = __builtin_nontemporal_load(a + i); s += v[0]; }
+  if (s == 1.0f) sink[blockIdx.x] = s;      // the buffer holds zeros: never stored, keeps the loads
+}
+__global__ void __launch_bounds__(256) k_stream_write(f32x4 *__restrict__ b, size_t n4)
+{
+  const size_t step = (size_t)gridDim.x * 256u;
+  const f32x4 z = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n4; i += step) __builtin_nontemporal_store(z, b + i);
+}
+__global__ void __launch_bounds__(256) k_stream_copy(const f32x4 *__restrict__ a, f32x4 *__restrict__ b, size_t n4)
+{
+  const size_t step = (size_t)gridDim.x * 256u;
+  size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+  for (; i + step < n4; i += 2 * step) {
+    const f32x4 v0 = __builtin_nontemporal_load(a + i), v1 = __builtin_nontemporal_load(a + i + step);
+    __builtin_nontemporal_store(v0, b + i); __builtin_nontemporal_store(v1, b + i + step);
+  }
+  for (; i < n4; i += step) __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
+}
+}
+
+// hand-written streaming bandwidth probe: mode 0 reads `bytes`, 1 writes them,
+// 2 copies them (read + write counted); `reps` timed launches after one
+// warm-up launch; gbs = bytes moved / s.  No reference counterpart
+// (measurement aid: the achievable ceiling beside the 8 TB/s peak).
+extern "C" int felbm_stream_probe(int device, int64_t bytes, int reps, int mode, double *gbs)
+{
+  if (!gbs || bytes < 16 || reps <= 0 || mode < 0 || mode > 2) { fm::set_error("invalid stream probe arguments"); return FELBM_EINVAL; }
+  if (hipSetDevice(device) != hipSuccess) { fm::set_error("hipSetDevice failed"); return FELBM_EHIP; }
+  const size_t n4 = (size_t)bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  float *sink = nullptr;
+  int nsm = 256;
+  (void)hipDeviceGetAttribute(&nsm, hipDeviceAttributeMultiprocessorCount, device);
+  const int grid = nsm * 16;
+  if (hipMalloc(&a, n4 * 16) != hipSuccess) { fm::set_error("allocation failed"); return FELBM_ENOMEM; }
+  if (hipMalloc(&b, n4 * 16) != hipSuccess) { (void)hipFree(a); fm::set_error("allocation failed"); return FELBM_ENOMEM; }
+  if (hipMalloc((void **)&sink, (size_t)grid * 4) != hipSuccess) { (void)hipFree(a); (void)hipFree(b); fm::set_error("allocation failed"); return FELBM_ENOMEM; }
+  hipStream_t s; hipEvent_t e0, e1;
+  int rc = FELBM_OK;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) rc = FELBM_EHIP;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = FELBM_EHIP;
+  if (!rc) {
+    (void)hipMemsetAsync(a, 0, n4 * 16, s);
+    auto launch = [&]() {
+      if (mode == 0) hipLaunchKernelGGL(fm::k_stream_read, dim3(grid), dim3(256), 0, s, (const fm::f32x4 *)a, n4, sink);
+      else if (mode == 1) hipLaunchKernelGGL(fm::k_stream_write, dim3(grid), dim3(256), 0, s, (fm::f32x4 *)b, n4);
+      else hipLaunchKernelGGL(fm::k_stream_copy, dim3(grid), dim3(256), 0, s, (const fm::f32x4 *)a, (fm::f32x4 *)b, n4);
+    };
+    launch();
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; ++r) launch();
+    (void)hipEventRecord(e1, s);
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) rc = FELBM_EHIP;
+    float ms = 0; (void)hipEventElapsedTime(&ms, e0, e1);
+    const double moved = (double)n4 * 16.0 * (mode == 2 ? 2.0 : 1.0) * reps;
+    *gbs = ms > 0 ? moved / (ms * 1e-3) / 1e9 : 0.0;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipStreamDestroy(s);
+  }
+  (void)hipFree(a); (void)hipFree(b); (void)hipFree(sink);
+  if (rc) fm::set_error("stream probe failed");
+  return rc;
+}
+
 extern "C" int felbm_copy_probe(int device, int64_t bytes, int reps, double *gbs)
 {
   // device-to-device copy of `bytes`, `reps` times: the achievable streaming

@@ -29,9 +29,9 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 
 template <bool MRT, bool HBB, bool OPEN>
 __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restrict__ D,
-                                                     const int2 *__restrict__ map, unsigned T, int stats)
+                                                     const int2 *__restrict__ map, int sdi, unsigned T, int stats)
 {
-  int2 bm = map[logical_block()];
+  const int2 bm = sdi >= 0 ? make_int2(sdi, logical_block()*(int)blockDim.x) : map[logical_block()];
   if (bm.x < 0) return;
   const Inst<float> &d = D[bm.x];
   const unsigned cur = (T ^ (unsigned)d.par0) & 1u;
@@ -39,12 +39,11 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
   const bool live = i < d.n;
   Fx s0 = {0,0,0}, s1 = {0,0,0}, s2 = {0,0,0}, s3 = {0,0,0}, s4 = {0,0,0};
   unsigned long long vmax = 0;
-  // this step's mass-correction lambda (k_mass_lambda), loaded before the
-  // stencil loads so that waiting for it does not wait for them
+  // this step's mass-correction lambda, which k_mass_lambda left in the
+  // descriptor: it arrives with the descriptor's scalar loads
   float lam = 0.0f;
   if (d.mass_on) {
-    const unsigned long long ap = ldg(d.acc, ACC_FLAGS + 4), lb = ldg(d.acc, ACC_FLAGS + 3);
-    lam = ap ? (float)__longlong_as_double((long long)lb) : 0.0f;
+    lam = d.lam_on ? (float)d.lam : 0.0f;
   }
   if (threadIdx.x == 0 && bm.y == 0) {
     if (d.mass_on) stg(d.acc, ACC_FLAGS, 1ull);
@@ -61,6 +60,15 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
     Nbhd nb; make_nbhd(d, x, y, z, nb);
     unsigned vo[Q], fl, so, fl2;
     ring_f<float, OPEN, 1, false>(d, rW, nb, io, vo, fl, so);
+    // The streamed g, read with the non-temporal hint: its lines are not read
+    // again this step, and evicting them first keeps the neighbour records of
+    // the +-2 planes in L2 (0.826 -> 0.766 ms).  Issued as soon as the ring-1
+    // offsets are known, so the HBM latency of the gather overlaps the ring-2
+    // row windows (0.724 -> 0.720 ms); issuing it together with the record
+    // loads instead cost 10%.
+    float gk[Q];
+    #pragma unroll
+    for (int k = 0; k < Q; ++k) gk[k] = bld_nt<float>(rGi, pull_off(k, vo, fl, io, nsb), pull_soff(k, nsb));
     // open-boundary node; a buffer node does not collide and stores the
     // equilibrium the reference writes into it (k_open_bnd_buf)
     int eb = -1;
@@ -156,13 +164,9 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
         Ph[o] = a2*bco + a4*bpo + hmob*lN[o];
         dc2[m] = cdc[m] + bcm; dc2[o] = cdc[o] + bco;
       }
-      float gk[Q];
       float jx = 0, jy = 0, jz = 0;
       #pragma unroll
-      // the streamed g, read with the non-temporal hint: its lines are not
-      // read again this step, and evicting them first keeps the neighbour
-      // records of the +-2 planes in L2 (0.826 -> 0.766 ms)
-      for (int k = 0; k < Q; ++k) { gk[k] = bld_nt<float>(rGi, pull_off(k, vo, fl, io, nsb), pull_soff(k, nsb)); acc_vec<float>(k, gk[k], jx, jy, jz); }
+      for (int k = 0; k < Q; ++k) acc_vec<float>(k, gk[k], jx, jy, jz);
       const float f_dcx = 0.5f*(e_dcx + b_cx), f_dcy = 0.5f*(e_dcy + b_cy), f_dcz = 0.5f*(e_dcz + b_cz);
       const float f_dpx = 0.5f*(e_dpx + b_px), f_dpy = 0.5f*(e_dpy + b_py), f_dpz = 0.5f*(e_dpz + b_pz);
       const float inv = 3.0f*irho;

@@ -794,6 +794,7 @@ private:
   static constexpr int kBS[3] = { 64, 128, 256 };
   int2 *d_var_[3][8] = {};
   int nvar_[3][8] = {};
+  int vsdi_[3][8] = {};             // the one descriptor of a variant map, or -1 (several instances)
   int bsi_[5] = { 2, 2, 2, 2, 2 };  // shape index of passes 1-4 (moments, c, mu, collide)
   int map_cap_ = 0;
   int nmap_ = 0;
@@ -868,6 +869,12 @@ private:
         for (int b = 0; b * kBS[bi] < slots_[q].n; ++b) var[bi][key].push_back(make_int2(di, b * kBS[bi]));
     }
     auto pad = [](std::vector<int2> &v) { while (v.size() % 8) v.push_back(make_int2(-1, 0)); };
+    for (int bi = 0; bi < 3; ++bi)
+      for (int k = 0; k < 8; ++k) {
+        int one = var[bi][k].empty() ? -1 : var[bi][k][0].x;
+        for (const int2 &e : var[bi][k]) if (e.x != one) { one = -1; break; }
+        vsdi_[bi][k] = one;
+      }
     pad(all); for (auto &vb : var) for (auto &v : vb) pad(v);
     nmap_ = (int)all.size();
     int need = nmap_;
@@ -947,37 +954,37 @@ private:
   }
 
   template <bool MRT, bool HBB, bool OPEN>
-  void launch_collide(dim3 grid, const int2 *map, bool stats, int bs) {
+  void launch_collide(dim3 grid, const int2 *map, int sdi, bool stats, int bs) {
     if constexpr (sizeof(real) == 4)
-      hipLaunchKernelGGL((k_collide_f32<MRT, HBB, OPEN>), grid, dim3(bs), 0, stream_, (const Inst<float> *)d_desc_, map, T_, stats ? 1 : 0);
+      hipLaunchKernelGGL((k_collide_f32<MRT, HBB, OPEN>), grid, dim3(bs), 0, stream_, (const Inst<float> *)d_desc_, map, sdi, T_, stats ? 1 : 0);
     else
-      hipLaunchKernelGGL((k_collide<real, MRT, HBB, OPEN>), grid, dim3(bs), 0, stream_, d_desc_, map, T_, stats ? 1 : 0);
+      hipLaunchKernelGGL((k_collide<real, MRT, HBB, OPEN>), grid, dim3(bs), 0, stream_, d_desc_, map, sdi, T_, stats ? 1 : 0);
   }
   template <bool MRT, bool HBB, bool OPEN>
-  void launch_variant(int stage, dim3 grid, const int2 *map, bool stats, real *exp, int exp_inst, int bs) {
-    if (stage == 2) hipLaunchKernelGGL((k_cfield<real, HBB, OPEN>), grid, dim3(bs), 0, stream_, d_desc_, map, T_, exp, exp_inst);
-    else if (stage == 3) hipLaunchKernelGGL((k_mufield<real, HBB, OPEN>), grid, dim3(bs), 0, stream_, d_desc_, map, T_);
-    else launch_collide<MRT, HBB, OPEN>(grid, map, stats, bs);
+  void launch_variant(int stage, dim3 grid, const int2 *map, int sdi, bool stats, real *exp, int exp_inst, int bs) {
+    if (stage == 2) hipLaunchKernelGGL((k_cfield<real, HBB, OPEN>), grid, dim3(bs), 0, stream_, d_desc_, map, sdi, T_, exp, exp_inst);
+    else if (stage == 3) hipLaunchKernelGGL((k_mufield<real, HBB, OPEN>), grid, dim3(bs), 0, stream_, d_desc_, map, sdi, T_);
+    else launch_collide<MRT, HBB, OPEN>(grid, map, sdi, stats, bs);
   }
-  void launch_key(int key, int stage, dim3 grid, const int2 *map, bool stats, real *exp = nullptr, int exp_inst = -1,
+  void launch_key(int key, int stage, dim3 grid, const int2 *map, int sdi, bool stats, real *exp = nullptr, int exp_inst = -1,
                   int bs = 256) {
     switch (key) {
-      case 0: launch_variant<false, false, false>(stage, grid, map, stats, exp, exp_inst, bs); break;
-      case 1: launch_variant<true,  false, false>(stage, grid, map, stats, exp, exp_inst, bs); break;
-      case 2: launch_variant<false, true,  false>(stage, grid, map, stats, exp, exp_inst, bs); break;
-      case 3: launch_variant<true,  true,  false>(stage, grid, map, stats, exp, exp_inst, bs); break;
-      case 4: launch_variant<false, false, true >(stage, grid, map, stats, exp, exp_inst, bs); break;
-      case 5: launch_variant<true,  false, true >(stage, grid, map, stats, exp, exp_inst, bs); break;
-      case 6: launch_variant<false, true,  true >(stage, grid, map, stats, exp, exp_inst, bs); break;
-      default: launch_variant<true,  true,  true >(stage, grid, map, stats, exp, exp_inst, bs); break;
+      case 0: launch_variant<false, false, false>(stage, grid, map, sdi, stats, exp, exp_inst, bs); break;
+      case 1: launch_variant<true,  false, false>(stage, grid, map, sdi, stats, exp, exp_inst, bs); break;
+      case 2: launch_variant<false, true,  false>(stage, grid, map, sdi, stats, exp, exp_inst, bs); break;
+      case 3: launch_variant<true,  true,  false>(stage, grid, map, sdi, stats, exp, exp_inst, bs); break;
+      case 4: launch_variant<false, false, true >(stage, grid, map, sdi, stats, exp, exp_inst, bs); break;
+      case 5: launch_variant<true,  false, true >(stage, grid, map, sdi, stats, exp, exp_inst, bs); break;
+      case 6: launch_variant<false, true,  true >(stage, grid, map, sdi, stats, exp, exp_inst, bs); break;
+      default: launch_variant<true,  true,  true >(stage, grid, map, sdi, stats, exp, exp_inst, bs); break;
     }
   }
 
   // the moments pass gathers through the neighbour rings: open (non-periodic)
   // variants need the in-box checks
-  void launch_moments(int key, dim3 grid, const int2 *map, int bs = 256) {
-    if (key & 4) hipLaunchKernelGGL((k_moments<real, true>), grid, dim3(bs), 0, stream_, d_desc_, map, T_);
-    else hipLaunchKernelGGL((k_moments<real, false>), grid, dim3(bs), 0, stream_, d_desc_, map, T_);
+  void launch_moments(int key, dim3 grid, const int2 *map, int sdi, int bs = 256) {
+    if (key & 4) hipLaunchKernelGGL((k_moments<real, true>), grid, dim3(bs), 0, stream_, d_desc_, map, sdi, T_);
+    else hipLaunchKernelGGL((k_moments<real, false>), grid, dim3(bs), 0, stream_, d_desc_, map, sdi, T_);
   }
 
   int launch_step(bool stats, bool timed = false) {
@@ -985,7 +992,7 @@ private:
     dim3 blk(256);
     if (timed) FM_HIP(hipEventRecord(ev_[0], stream_));
     int bi = bsi_[1];
-    for (int k = 0; k < 8; ++k) if (nvar_[bi][k]) { launch_moments(k, dim3(nvar_[bi][k]), d_var_[bi][k], kBS[bi]); FM_LAUNCH(); }
+    for (int k = 0; k < 8; ++k) if (nvar_[bi][k]) { launch_moments(k, dim3(nvar_[bi][k]), d_var_[bi][k], vsdi_[bi][k], kBS[bi]); FM_LAUNCH(); }
     if (n_ent_all_) {
       hipLaunchKernelGGL(k_open_fields<real>, dim3((n_ent_all_ + 255) / 256), blk, 0, stream_, d_desc_, d_ent_all_, n_ent_all_, T_);
       FM_LAUNCH();
@@ -998,10 +1005,10 @@ private:
     }
     if (timed) FM_HIP(hipEventRecord(ev_[1], stream_));
     bi = bsi_[2];
-    for (int k = 0; k < 8; ++k) if (nvar_[bi][k]) { launch_key(k, 2, dim3(nvar_[bi][k]), d_var_[bi][k], stats, nullptr, -1, kBS[bi]); FM_LAUNCH(); }
+    for (int k = 0; k < 8; ++k) if (nvar_[bi][k]) { launch_key(k, 2, dim3(nvar_[bi][k]), d_var_[bi][k], vsdi_[bi][k], stats, nullptr, -1, kBS[bi]); FM_LAUNCH(); }
     if (timed) FM_HIP(hipEventRecord(ev_[2], stream_));
     bi = bsi_[3];
-    for (int k = 0; k < 8; ++k) if (nvar_[bi][k]) { launch_key(k, 3, dim3(nvar_[bi][k]), d_var_[bi][k], stats, nullptr, -1, kBS[bi]); FM_LAUNCH(); }
+    for (int k = 0; k < 8; ++k) if (nvar_[bi][k]) { launch_key(k, 3, dim3(nvar_[bi][k]), d_var_[bi][k], vsdi_[bi][k], stats, nullptr, -1, kBS[bi]); FM_LAUNCH(); }
     // tracers move on this step's u snapshot before the collision pass
     // advances the step counter they key their draws with
     for (size_t q = 0; q < slots_.size(); ++q) {
@@ -1018,7 +1025,7 @@ private:
     }
     if (timed) FM_HIP(hipEventRecord(ev_[3], stream_));
     bi = bsi_[4];
-    for (int k = 0; k < 8; ++k) if (nvar_[bi][k]) { launch_key(k, 4, dim3(nvar_[bi][k]), d_var_[bi][k], stats, nullptr, -1, kBS[bi]); FM_LAUNCH(); }
+    for (int k = 0; k < 8; ++k) if (nvar_[bi][k]) { launch_key(k, 4, dim3(nvar_[bi][k]), d_var_[bi][k], vsdi_[bi][k], stats, nullptr, -1, kBS[bi]); FM_LAUNCH(); }
     if (timed) FM_HIP(hipEventRecord(ev_[4], stream_));
     ++T_;
     return FELBM_OK;
@@ -1069,7 +1076,7 @@ private:
     const int di = slot_desc_[&S - slots_.data()];
     const size_t ns = (size_t)S.ns;
     dim3 blk(256), grid(S.dmap_len);
-    launch_moments(variant(S), grid, S.dmap); FM_LAUNCH();
+    launch_moments(variant(S), grid, S.dmap, di); FM_LAUNCH();
     if (S.obc && !S.entries.empty()) {
       const int ne = (int)S.entries.size();
       hipLaunchKernelGGL(k_open_fields<real>, dim3((ne + 255) / 256), blk, 0, stream_, d_desc_, d_ent_all_ + S.e0, ne, T_);
@@ -1079,7 +1086,7 @@ private:
     }
     hipLaunchKernelGGL(k_mass_lambda<real>, dim3(1), dim3(64), 0, stream_, d_desc_, di, 1, T_);
     FM_LAUNCH();
-    launch_key(variant(S), 2, grid, S.dmap, false, S.d.craw + export_off_ * ns, di); FM_LAUNCH();
+    launch_key(variant(S), 2, grid, S.dmap, di, false, S.d.craw + export_off_ * ns, di); FM_LAUNCH();
     // the step's own moments pass must find this parity's partials at zero
     unsigned cur = cur_of(S);
     FM_HIP(hipMemsetAsync(S.d.acc + ACC_MASS + cur * 5 * NSLOT, 0, 5 * NSLOT * 8, stream_));

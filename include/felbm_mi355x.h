@@ -222,6 +222,12 @@ int  felbm_set_block_sizes(felbm_engine *e, const int *bs4);
  * byte counters.  No reference counterpart (measurement aid). */
 int  felbm_copy_probe(int device, int64_t bytes, int reps, double *gbs);
 
+/* hand-written streaming probe: mode 0 reads `bytes`, 1 writes them, 2 copies
+ * them (16 bytes per lane, non-temporal, grid-stride), `reps` timed launches;
+ * gbs = bytes moved / s.  The achievable HBM ceiling the kernels are compared
+ * with.  No reference counterpart (measurement aid). */
+int  felbm_stream_probe(int device, int64_t bytes, int reps, int mode, double *gbs);
+
 const char *felbm_last_error(void);
 const char *felbm_version(void);
 
