@@ -66,7 +66,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="porous_300")
-    ap.add_argument("--cpu-steps", type=int, default=int(os.environ.get("FELBM_CPU_STEPS", "24")))
+    ap.add_argument("--cpu-steps", type=int, default=int(os.environ.get("FELBM_CPU_STEPS", "40")),
+                    help="steps of the FP64 OpenMP oracle on the same lattice (a bounded sample, ~30 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-steps", type=int, default=1000,
                     help="steps of the 48-instance batched sweep (0 skips it)")
@@ -160,10 +161,11 @@ def main():
         eng.tracers_init(pos, dm, seed=3, substeps=1, velocity_skip=1)
         eng.step(5)
         eng.batch.synchronize()
+        tr_steps = max(args.steps, 200)
         t1 = time.perf_counter()
-        eng.step(args.steps)
+        eng.step(tr_steps)
         eng.batch.synchronize()
-        tr_mlups = n * args.steps / (time.perf_counter() - t1) / 1e6
+        tr_mlups = n * tr_steps / (time.perf_counter() - t1) / 1e6
         eng.tracers_init(np.zeros((0, 3)), np.zeros(0))
         print("[bench] with 12,000 tracers: %.1f MLUPS" % tr_mlups, file=sys.stderr, flush=True)
     print("[bench] %.1f MLUPS, kernels %s" % (mlups, json.dumps(kern)), file=sys.stderr, flush=True)
@@ -207,6 +209,12 @@ def main():
         from felbm_gpu_amd import serving
         sweep = serving.run_sweep(W.sweep_instances(48, 1000), args.sweep_steps, "f32", log=log, check_fp64=True)
         log("sweep: %s" % json.dumps(sweep))
+        # the same sweep with every instance at edge 64 (about 1e5 sites each):
+        # the regime where one launch per pass over the whole batch matters
+        small = serving.run_sweep(W.sweep_instances(48, 1000, edges=(64,)), args.sweep_steps, "f32", log=log)
+        sweep["small_64"] = {k: small[k] for k in ("instances", "sites", "steps", "batched_MLUPS", "sequential_MLUPS",
+                                                  "speedup", "retired_nonfinite")}
+        log("sweep 48 x 64^3: %s" % json.dumps(sweep["small_64"]))
     served = None
     if args.serve > 0:
         eng.free()
@@ -221,6 +229,9 @@ def main():
         print("[bench] cpu baseline (%d steps)" % args.cpu_steps, file=sys.stderr, flush=True)
         eng.free()
         cpu = cpu_baseline(args.workload, w, args.cpu_steps)
+        # the GPU rates on the same lattice beside it: FP64 (like for like with
+        # the FP64 oracle) over the FP32-vs-FP64 run, FP32 the headline
+        cpu["gpu_same_lattice_MLUPS"] = {"f32": mlups, "f64": f32_err["MLUPS_over_run"]["f64"] if f32_err else None}
 
     out = {
         "metric": "MLUPS (300^3 fused MRT) + batched-sweep MLUPS and p50/p99 solve latency",

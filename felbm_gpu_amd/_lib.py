@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfelbm_mi355x.so")
 
 FELBM_OK = 0
+FELBM_ENONFINITE = -6
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ESTATE", -5: "EUNSUPPORTED", -6: "ENONFINITE"}
 
 
@@ -21,7 +22,8 @@ class FelbmError(RuntimeError):
 
 class Geometry(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("periodic", C.c_int * 3),
-                ("use_halfway_bb", C.c_int), ("solid", C.POINTER(C.c_uint8))]
+                ("use_halfway_bb", C.c_int), ("solid", C.POINTER(C.c_uint8)),
+                ("site_xyz", C.POINTER(C.c_int32)), ("n_sites", C.c_int64)]
 
 
 class Params(C.Structure):

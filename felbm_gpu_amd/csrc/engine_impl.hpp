@@ -44,6 +44,11 @@ struct Slot {
   int2 *dmap = nullptr;           // single-instance block map
   int dmap_len = 0;
   int stats_valid = 0;
+  // the caller's site numbering (felbm_geometry.site_xyz): caller site i is
+  // engine site perm[i], engine site e is caller site iperm[e] (device copy
+  // d_iperm for the table export); empty when it is the engine's own order
+  std::vector<int> perm, iperm;
+  int *d_iperm = nullptr;
   // open boundaries
   int obc = 0;
   char *ob_base = nullptr;        // ent [n] + ov [entries*8]
@@ -123,6 +128,32 @@ public:
     words.push_back(0);
     if (cnt == 0) { set_error("geometry has no fluid sites"); return FELBM_EINVAL; }
     if (cnt >= (int64_t)1 << 31) { set_error("more than 2^31 sites"); return FELBM_EUNSUPPORTED; }
+    if (g->site_xyz) {
+      // the caller's numbering: each listed voxel must be fluid and listed once
+      if (g->n_sites != cnt) {
+        set_error("site list has " + std::to_string(g->n_sites) + " entries, the mask has " + std::to_string(cnt) +
+                  " fluid voxels");
+        return FELBM_EINVAL;
+      }
+      S.perm.assign((size_t)cnt, -1); S.iperm.assign((size_t)cnt, -1);
+      bool ident = true;
+      for (int64_t i = 0; i < cnt; ++i) {
+        const int x = g->site_xyz[3 * i], y = g->site_xyz[3 * i + 1], z = g->site_xyz[3 * i + 2];
+        const std::string at = " (site " + std::to_string(i) + " at " + std::to_string(x) + ", " + std::to_string(y) +
+                               ", " + std::to_string(z) + ")";
+        if (x < 0 || x >= g->nx || y < 0 || y >= g->ny || z < 0 || z >= g->nz) {
+          set_error("site outside the box" + at); return FELBM_EINVAL;
+        }
+        const uint64_t w = words[(size_t)((((int64_t)z * g->ny) + y) * wpr + (x >> 5))];
+        const unsigned b = (unsigned)x & 31u;
+        if (!((w >> b) & 1u)) { set_error("site on a solid voxel" + at); return FELBM_EINVAL; }
+        const int e = (int)(w >> 32) + __builtin_popcount((unsigned)w & ((1u << b) - 1u));
+        if (S.iperm[(size_t)e] >= 0) { set_error("voxel listed twice" + at); return FELBM_EINVAL; }
+        S.perm[(size_t)i] = e; S.iperm[(size_t)e] = (int)i;
+        ident = ident && e == (int)i;
+      }
+      if (ident) { S.perm.clear(); S.iperm.clear(); }
+    }
     S.n = (int)cnt; S.ns = (int)align_up((size_t)cnt, 64); S.wpr = wpr;
     if ((int64_t)g->ny * wpr >= ((int64_t)1 << 24)) {     // row offsets use 24-bit multiplies
       set_error("box too large: ny * ceil(nx/32) must stay below 2^24"); return FELBM_EUNSUPPORTED;
@@ -141,12 +172,15 @@ public:
     size_t o_w = take(words.size() * 8), o_c = take(coords.size() * 4);
     size_t o_a = take(sizeof(unsigned long long) * ACC_WORDS);
     size_t o_m = take(sizeof(int2) * (size_t)(align_up((size_t)S.nblocks, 8)));
+    const size_t o_p = S.iperm.empty() ? 0 : take(S.iperm.size() * sizeof(int));
     S.bytes = off;
     if (hipMalloc((void **)&S.base, off) != hipSuccess) { set_error("device allocation failed"); return FELBM_ENOMEM; }
     {
       hipError_t he = hipMemsetAsync(S.base, 0, off, stream_);
       if (he == hipSuccess) he = hipMemcpyAsync(S.base + o_w, words.data(), words.size() * 8, hipMemcpyHostToDevice, stream_);
       if (he == hipSuccess) he = hipMemcpyAsync(S.base + o_c, coords.data(), coords.size() * 4, hipMemcpyHostToDevice, stream_);
+      if (he == hipSuccess && !S.iperm.empty())
+        he = hipMemcpyAsync(S.base + o_p, S.iperm.data(), S.iperm.size() * sizeof(int), hipMemcpyHostToDevice, stream_);
       if (he == hipSuccess) he = hipStreamSynchronize(stream_);
       if (he != hipSuccess) {
         (void)hipStreamSynchronize(stream_);
@@ -170,6 +204,7 @@ public:
     d.mu = f + F_MU * ns; d.clo = f + F_CLO * ns;
     export_off_ = 9;   // exports at f + 9*ns .. 12*ns
     d.acc = (unsigned long long *)(S.base + o_a);
+    S.d_iperm = S.iperm.empty() ? nullptr : (int *)(S.base + o_p);
     set_params(S);
     // admission parity: g0 holds the state at the current step counter
     S.par0 = (int)(T_ & 1u); d.par0 = S.par0;
@@ -230,10 +265,10 @@ public:
       if (pass && p_ref<real>() != 0.0) {        // FP32 g slots hold g_k - w_k (p_ref, kernels.hpp)
         for (int k = 0; k < 19; ++k) {
           const double off = (double)wf32(k) * p_ref<real>();
-          for (size_t i = 0; i < n; ++i) t[k * ns + i] = (real)g_flush((float)(src[k * n + i] - off));
+          for (size_t i = 0; i < n; ++i) t[k * ns + eng(S, i)] = (real)g_flush((float)(src[k * n + i] - off));
         }
       } else {
-        for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) t[k * ns + i] = (real)src[k * n + i];
+        for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) t[k * ns + eng(S, i)] = (real)src[k * n + i];
       }
       real *dst = pass ? (cur ? S.d.g1 : S.d.g0) : S.d.h;
       FM_HIP(hipMemcpyAsync(stage, t.data(), t.size() * sizeof(real), hipMemcpyHostToDevice, stream_));
@@ -271,14 +306,14 @@ public:
     std::vector<real> t(19 * ns);
     FM_HIP(hipMemcpyAsync(t.data(), stage, t.size() * sizeof(real), hipMemcpyDeviceToHost, stream_));
     FM_HIP(hipStreamSynchronize(stream_));
-    for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) h[k * n + i] = (double)t[k * ns + i];
+    for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) h[k * n + i] = (double)t[k * ns + eng(S, i)];
     hipLaunchKernelGGL(k_gather_copy<real>, dim3(nb), dim3(256), 0, stream_, S.d, (const real *)(cur ? S.d.g1 : S.d.g0), stage);
     FM_LAUNCH();
     FM_HIP(hipMemcpyAsync(t.data(), stage, t.size() * sizeof(real), hipMemcpyDeviceToHost, stream_));
     FM_HIP(hipStreamSynchronize(stream_));
     for (int k = 0; k < 19; ++k) {
       const double off = p_ref<real>() != 0.0 ? (double)wf32(k) * p_ref<real>() : 0.0;
-      for (size_t i = 0; i < n; ++i) g[k * n + i] = (double)t[k * ns + i] + off;
+      for (size_t i = 0; i < n; ++i) g[k * n + i] = (double)t[k * ns + eng(S, i)] + off;
     }
     return FELBM_OK;
   }
@@ -407,9 +442,10 @@ public:
     FM_HIP(hipMemcpyAsync(cp.data(), S.d.c, cp.size() * sizeof(real), hipMemcpyDeviceToHost, stream_));   // [c, p*] pairs
     FM_HIP(hipStreamSynchronize(stream_));
     for (size_t i = 0; i < n; ++i) {
-      out[i] = (double)t[i]; out[n + i] = (double)cp[2 * i];
-      out[2 * n + i] = (double)t[ns + i]; out[3 * n + i] = (double)t[2 * ns + i]; out[4 * n + i] = (double)t[3 * ns + i];
-      out[5 * n + i] = (double)cp[2 * i + 1] + p_ref<real>();    // p* is stored as p - p_ref
+      const size_t e = eng(S, i);
+      out[i] = (double)t[e]; out[n + i] = (double)cp[2 * e];
+      out[2 * n + i] = (double)t[ns + e]; out[3 * n + i] = (double)t[2 * ns + e]; out[4 * n + i] = (double)t[3 * ns + e];
+      out[5 * n + i] = (double)cp[2 * e + 1] + p_ref<real>();    // p* is stored as p - p_ref
     }
     return FELBM_OK;
   }
@@ -432,6 +468,17 @@ public:
       set_error("split and stripes inlet modes need the inlet coordinates"); return FELBM_EINVAL;
     }
     const int n = S.n;
+    // node lists in the caller's numbering -> engine sites (out-of-range
+    // indices are passed on and refused below)
+    std::vector<int32_t> in_e, out_e, bv_e, br_e;
+    if (!S.perm.empty()) {
+      auto map_list = [&](const int32_t *a, int cnt, std::vector<int32_t> &o) {
+        o.resize((size_t)cnt);
+        for (int q = 0; q < cnt; ++q) o[(size_t)q] = (a[q] >= 0 && a[q] < n) ? S.perm[(size_t)a[q]] : -1;
+      };
+      map_list(inlet, n_in, in_e); map_list(outlet, n_out, out_e); map_list(bv, n_buf, bv_e); map_list(br, n_buf, br_e);
+      inlet = in_e.data(); outlet = out_e.data(); bv = bv_e.data(); br = br_e.data();
+    }
     std::vector<int> ent((size_t)n, -1), kind((size_t)n, 3), pos((size_t)n, -1);
     std::vector<ObEntry> E;
     const bool in_proc = ob->use_in_vel || ob->use_in_prs || ob->use_in_fluid;
@@ -503,7 +550,7 @@ public:
     int *p[10]; size_t o = 0;
     for (int q = 0; q < 10; ++q) { p[q] = buf + o; o += sz[q]; }
     hipLaunchKernelGGL(k_export_tables<real>, dim3((S.n + 255) / 256), dim3(256), 0, stream_, S.d,
-                       p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9]);
+                       p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], (const int *)S.d_iperm);
     hipError_t le = hipGetLastError();
     std::vector<int> h(tot);
     if (le == hipSuccess) le = hipMemcpyAsync(h.data(), buf, tot * 4, hipMemcpyDeviceToHost, stream_);
@@ -536,7 +583,7 @@ public:
     int *p[10]; size_t o = 0;
     for (int q = 0; q < 10; ++q) { p[q] = buf + o; o += sz[q]; }
     hipLaunchKernelGGL(k_export_tables<real>, dim3((S.n + 255) / 256), dim3(256), 0, stream_, S.d,
-                       p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9]);
+                       p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], (const int *)S.d_iperm);
     FM_LAUNCH();
     for (int q = 0; q < 10; ++q)
       if (tabs[q]) FM_HIP(hipMemcpyAsync(tabs[q], p[q], sz[q] * sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -823,6 +870,8 @@ private:
     return FELBM_OK;
   }
   unsigned cur_of(const Slot<real> &S) const { return (T_ ^ (unsigned)S.par0) & 1u; }
+  // engine site of the caller's site i
+  static size_t eng(const Slot<real> &S, size_t i) { return S.perm.empty() ? i : (size_t)S.perm[i]; }
 
   void set_params(Slot<real> &S) {
     const felbm_params &p = S.prm;

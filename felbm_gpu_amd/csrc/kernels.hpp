@@ -232,12 +232,14 @@ __device__ __forceinline__ void bst(double v, rsrc_t r, unsigned vo, unsigned so
 // field block layout (ns values each): c_raw; [p0, jx, jy, jz] interleaved per
 // site (4 ns); [c, p*] interleaved per site (2 ns); mu; the FP32 remainder
 // Sigma h - c_raw of the moments pass (unused in FP64); then 4 x ns
-// holding either the per-site records [c, p*, lapmu, mu] the mu-stencil pass
-// writes for the collision (one vector load per neighbour), or, between steps,
-// the exported rho, ux, uy, uz of a field download.  A pass reads its own
+// holding either what the mu-stencil pass writes for the collision -- FP32:
+// the per-site records [c, p*, lapmu, mu] (one vector load per neighbour);
+// FP64: lapmu alone (F_LMU), the collision reading the [c, p*] pairs, whose 16
+// bytes a record would double -- or, between steps, the exported rho, ux, uy,
+// uz of a field download.  A pass reads its own
 // site's inputs with one vector access per group: every vector memory
 // instruction costs the address unit the same whatever its width.
-enum { F_CRAW = 0, F_PJ = 1, F_CP = 5, F_MU = 7, F_CLO = 8, F_REC = 9, F_COUNT = 13 };
+enum { F_CRAW = 0, F_PJ = 1, F_CP = 5, F_MU = 7, F_CLO = 8, F_REC = 9, F_LMU = 9, F_COUNT = 13 };
 
 // record loads/stores: (c, p*) or (c, p*, lapmu, mu) of one site, vo = site
 // byte offset in a scalar field (i * sizeof(real)); the record sits at 4*vo
@@ -329,20 +331,6 @@ __device__ __forceinline__ Rec4<float> bld_rec4(const float *, rsrc_t r, unsigne
   const f32x4 f = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(vo*4u), (int)so, 0));
   return { f[0], f[1], f[2], f[3] };
 }
-__device__ __forceinline__ Rec2<double> bld_rec2(const double *, rsrc_t r, unsigned vo, unsigned so)
-{
-  auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(vo*4u), (int)so, 0);
-  typedef __attribute__((ext_vector_type(2))) unsigned u2;
-  return { __builtin_bit_cast(double, u2{v[0], v[1]}), __builtin_bit_cast(double, u2{v[2], v[3]}) };
-}
-__device__ __forceinline__ Rec4<double> bld_rec4(const double *, rsrc_t r, unsigned vo, unsigned so)
-{
-  auto a = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(vo*4u), (int)so, 0);
-  auto b = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(vo*4u + 16u), (int)so, 0);
-  typedef __attribute__((ext_vector_type(2))) unsigned u2;
-  return { __builtin_bit_cast(double, u2{a[0], a[1]}), __builtin_bit_cast(double, u2{a[2], a[3]}),
-           __builtin_bit_cast(double, u2{b[0], b[1]}), __builtin_bit_cast(double, u2{b[2], b[3]}) };
-}
 template <int AUX = 0>
 __device__ __forceinline__ void bst_rec4(float c, float p, float l, float m, rsrc_t r, unsigned vo, unsigned so)
 {
@@ -350,15 +338,6 @@ __device__ __forceinline__ void bst_rec4(float c, float p, float l, float m, rsr
   u4 v = { __builtin_bit_cast(unsigned, c), __builtin_bit_cast(unsigned, p),
            __builtin_bit_cast(unsigned, l), __builtin_bit_cast(unsigned, m) };
   __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(vo*4u), (int)so, AUX);
-}
-template <int AUX = 0>
-__device__ __forceinline__ void bst_rec4(double c, double p, double l, double m, rsrc_t r, unsigned vo, unsigned so)
-{
-  typedef __attribute__((ext_vector_type(2))) unsigned u2;
-  typedef __attribute__((ext_vector_type(4))) unsigned u4;
-  u2 cc = __builtin_bit_cast(u2, c), pp = __builtin_bit_cast(u2, p), ll = __builtin_bit_cast(u2, l), mm = __builtin_bit_cast(u2, m);
-  __builtin_amdgcn_raw_buffer_store_b128(u4{cc[0], cc[1], pp[0], pp[1]}, r, (int)(vo*4u), (int)so, 0);
-  __builtin_amdgcn_raw_buffer_store_b128(u4{ll[0], ll[1], mm[0], mm[1]}, r, (int)(vo*4u + 16u), (int)so, 0);
 }
 
 // ---- lattice lookup ---------------------------------------------------------------
@@ -981,10 +960,18 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
     for (int m = 1; m < Q; ++m)
       s += lap_term<real, HBB>((fl >> m) & 1u, (fl >> OPP[m]) & 1u, mn[m], mn[OPP[m]], m0, (real)0, (real)3.0*W<real>(m));
   }
-  // own [c, p*], read once: non-temporal, like the other single-use streams,
-  // so L2 keeps the stencil data (mu here, the records in the collision pass)
-  const Rec2<real> cp = bld_cp<2>((const real *)nullptr, rF, io, F_CP*nsb);
-  bst_rec4(cp.c, cp.p, s, m0, rF, io, F_REC*nsb);
+  if constexpr (sizeof(real) == 4) {
+    // own [c, p*], read once: non-temporal, like the other single-use streams,
+    // so L2 keeps the stencil data (mu here, the records in the collision pass)
+    const Rec2<real> cp = bld_cp<2>((const real *)nullptr, rF, io, F_CP*nsb);
+    bst_rec4(cp.c, cp.p, s, m0, rF, io, F_REC*nsb);
+  } else {
+    // FP64: lapmu alone; the collision takes c and p* from the [c, p*] pairs
+    // and mu from its field (one 16-byte and one 8-byte load per ring-1
+    // neighbour instead of two 16-byte record loads: mu pass 0.244 -> 0.181 ms,
+    // collision 2.23 -> 2.15 ms at 300^3)
+    bst(s, rF, io, F_LMU*nsb);
+  }
 }
 
 // Store one post-collision population k of the site at byte offset io into
@@ -1064,21 +1051,23 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
       }
     } else {
       real cN[Q], pN[Q], c2[Q], p2[Q], lN[Q];
-      const Rec4<real> r0 = bld_rec4((const real *)nullptr, rF, io, F_REC*nsb);
+      const Rec2<real> r0 = bld_cp((const real *)nullptr, rF, io, F_CP*nsb);
+      const real lmu = bld<real>(rF, io, F_LMU*nsb), mu = bld<real>(rF, io, F_MU*nsb);
       {
-        // a neighbour that is not a site takes the own record's values
+        // a neighbour that is not a site takes the own values
         unsigned vo2[Q];
         { unsigned so2; ring_f<real, OPEN, 2, false>(d, rW, nb, io, vo2, fl2, so2); }
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
-          const Rec4<real> r1 = bld_rec4((const real *)nullptr, rF, vo[m], F_REC*nsb);
-          const Rec2<real> r2 = bld_rec2((const real *)nullptr, rF, vo2[m], F_REC*nsb);
+          const Rec2<real> r1 = bld_cp((const real *)nullptr, rF, vo[m], F_CP*nsb);
+          const real l1 = bld<real>(rF, vo[m], F_LMU*nsb);
+          const Rec2<real> r2 = bld_cp((const real *)nullptr, rF, vo2[m], F_CP*nsb);
           const bool s1 = (fl >> m) & 1u, s2 = (fl2 >> m) & 1u;
-          cN[m] = s1 ? r1.c : r0.c; pN[m] = s1 ? r1.p : r0.p; lN[m] = s1 ? r1.l : r0.l;
+          cN[m] = s1 ? r1.c : r0.c; pN[m] = s1 ? r1.p : r0.p; lN[m] = s1 ? l1 : lmu;
           c2[m] = s2 ? r2.c : r0.c; p2[m] = s2 ? r2.p : r0.p;
         }
       }
-      const real c = r0.c, p = r0.p, mu = r0.m, lmu = r0.l;
+      const real c = r0.c, p = r0.p;
       const real drho = d.drho;
       const real CS2 = (real)(1.0/3.0), ALPHA = (real)3.0, GAMMA_C = (real)1.5, BETA_C = (real)4.5;
       const real rho = ovp ? ldg(ovp, 1) : d.rho1 + drho*c;
@@ -1384,17 +1373,24 @@ __global__ void __launch_bounds__(256) k_unstream(Inst<real> d, const real *in, 
 
 // reference-format stencil/streaming tables (multiphase_gpu.cuh:93-208)
 // regenerated from the engine's neighbour logic, for bitwise comparison
+// iperm (may be null): the caller's number of each engine site; tables are
+// then laid out and filled in the caller's numbering
 template <typename real>
 __global__ void __launch_bounds__(256) k_export_tables(Inst<real> d, int *cdm, int *cdp, int *bcase,
                                                        int *bc1, int *bc2, int *avgnext, int *lcase,
-                                                       int *lc1, int *lc2, int *src)
+                                                       int *lc1, int *lc2, int *src, const int *iperm)
 {
-  const int i = blockIdx.x*blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
+  const int e = blockIdx.x*blockDim.x + threadIdx.x;
+  if (e >= d.n) return;
   const int n = d.n;
-  int x, y, z; decode(d, i, x, y, z);
+  int x, y, z; decode(d, e, x, y, z);
   Nbhd nb; make_nbhd(d, x, y, z, nb);
-  int n1[Q], n2[Q]; neighbours1(d, nb, i, n1); neighbours2(d, nb, n2);
+  int n1[Q], n2[Q]; neighbours1(d, nb, e, n1); neighbours2(d, nb, n2);
+  if (iperm) {
+    #pragma unroll
+    for (int m = 0; m < Q; ++m) { if (n1[m] >= 0) n1[m] = iperm[n1[m]]; if (n2[m] >= 0) n2[m] = iperm[n2[m]]; }
+  }
+  const int i = iperm ? iperm[e] : e;
   for (int m = 1; m < Q; ++m) {
     int in = n1[m], ip = n1[OPP[m]];
     size_t idx = (size_t)(m - 1)*n + i;

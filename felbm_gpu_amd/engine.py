@@ -27,7 +27,10 @@ def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def _geom(solid, periodic=(1, 1, 1), use_halfway_bb=True):
+def _geom(solid, periodic=(1, 1, 1), use_halfway_bb=True, site_xyz=None):
+    """felbm_geometry of a solid mask; site_xyz [n][3] (x, y, z) is the caller's
+    site numbering (None: x-fastest order).  Returns the struct and the arrays
+    it points at (to be kept alive for the call)."""
     solid = np.ascontiguousarray(solid, dtype=np.uint8)
     nz, ny, nx = solid.shape
     g = _lib.Geometry()
@@ -36,7 +39,13 @@ def _geom(solid, periodic=(1, 1, 1), use_halfway_bb=True):
         g.periodic[a] = int(bool(periodic[a]))
     g.use_halfway_bb = int(bool(use_halfway_bb))
     g.solid = solid.ctypes.data_as(C.POINTER(C.c_uint8))
-    return g, solid
+    keep = [solid]
+    if site_xyz is not None:
+        xyz = np.ascontiguousarray(site_xyz, dtype=np.int32).reshape(-1, 3)
+        g.site_xyz = xyz.ctypes.data_as(C.POINTER(C.c_int32))
+        g.n_sites = xyz.shape[0]
+        keep.append(xyz)
+    return g, keep
 
 
 def _params(p):
@@ -69,8 +78,8 @@ class Batch:
         self.h = h
         self._keep = {}
 
-    def add(self, solid, params, switches, periodic=(1, 1, 1), use_halfway_bb=True):
-        g, arr = _geom(solid, periodic, use_halfway_bb)
+    def add(self, solid, params, switches, periodic=(1, 1, 1), use_halfway_bb=True, site_xyz=None):
+        g, arr = _geom(solid, periodic, use_halfway_bb, site_xyz)
         slot = C.c_int()
         check(self.lib.felbm_add_instance(self.h, C.byref(g), C.byref(_params(params)),
                                           C.byref(_switches(switches)), C.byref(slot)))
@@ -137,15 +146,18 @@ class MultiPhaseGPU:
         self.correct_mass = False
         self.solid = None
         self._tr = None          # host copy of the tracer set-up (dm, plane_id) for particle files
+        self.site_xyz = None
         self._stretch = False
 
-    # multiphase_gpu.cuh:75-351
-    def init(self, solid, params, switches, periodic=(1, 1, 1), use_halfway_bb=True):
-        self.slot = self.batch.add(solid, params, switches, periodic, use_halfway_bb)
+    # multiphase_gpu.cuh:75-351; site_xyz [n][3] is the caller's site numbering
+    # (SubDomain::idx_to_coords order, src/felbm_gpu_main.cu:124-143)
+    def init(self, solid, params, switches, periodic=(1, 1, 1), use_halfway_bb=True, site_xyz=None):
+        self.slot = self.batch.add(solid, params, switches, periodic, use_halfway_bb, site_xyz)
         self.n = self.batch.n_sites(self.slot)
         self.Vn = 19 * self.n
         self.correct_mass = bool(switches.get("correct_op_mass", False))
         self.solid = np.asarray(solid)
+        self.site_xyz = None if site_xyz is None else np.asarray(site_xyz, dtype=np.int32).reshape(-1, 3)
         return self
 
     @property
@@ -287,7 +299,8 @@ class MultiPhaseGPU:
 
     def write_geometry_h5(self, path, deflate=0):
         from . import h5io
-        h5io.write_geometry(path, h5io.geometry_coords(self.solid), deflate)
+        coords = h5io.geometry_coords(self.solid) if self.site_xyz is None else self.site_xyz
+        h5io.write_geometry(path, coords, deflate)
 
     def write_checkpoint(self, path, step):
         from . import h5io

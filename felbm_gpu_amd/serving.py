@@ -27,13 +27,17 @@ import time
 import numpy as np
 
 from . import workloads as W
+from ._lib import FELBM_ENONFINITE, FelbmError
 
 
-def instance_bytes(n_sites, precision):
+def instance_bytes(n_sites, precision, n_open=0):
     """Device bytes of one instance: h, g0, g1 (19 each), 13 fields, coords,
-    and the row words and accumulators (rounded up)."""
+    and the row words and accumulators (rounded up); with open boundaries also
+    the per-site entry codes (4 B per site) and 8 values per boundary node
+    (engine_impl.hpp set_open_bnd)."""
     R = 8 if precision in ("f64", 8) else 4
-    return int(n_sites * (70 * R + 4) + (1 << 20))
+    ob = (4 * n_sites + 8 * R * max(1, n_open) + 512) if n_open else 0
+    return int(n_sites * (70 * R + 4) + ob + (1 << 20))
 
 
 # ---- parameter sweep ----------------------------------------------------------------
@@ -95,36 +99,68 @@ def run_sweep(specs, steps, precision="f32", graph_chunk=50, log=None, check_fp6
         log("sweep: autotuned sites per workgroup %s" % tune["block_sizes"])
     b.step(2)
     b.synchronize()
+    # each check interval (2 graph chunks): graph replays, then two eager
+    # steps the last of which fills the exact monitoring sums, copied back for
+    # every instance at once; an instance whose sums report non-finite fields
+    # is retired (its parameter point left the method's stable range) and the
+    # batch goes on without it
+    check = 2 * graph_chunk
+    steps_of = [0] * len(insts)
+    live = list(range(len(insts)))
+    nonfinite = []
     t0 = time.perf_counter()
-    b.step_graph(graph_chunk, steps // graph_chunk)
+    done = 0
+    while done < steps and live:
+        k = min(check, steps - done)
+        if k > 2:
+            b.step_graph(k - 2, 1)
+        b.step(2, stats_last=True)
+        done += k
+        for j in live:
+            steps_of[j] += k
+        insts[live[0]].flow_stats_launch()
+        for j in list(live):
+            try:
+                insts[j].flow_stats_collect()
+            except FelbmError as err:
+                if err.code != FELBM_ENONFINITE:
+                    raise
+                insts[j].free()
+                live.remove(j)
+                nonfinite.append(j)
     b.synchronize()
     t_batched = time.perf_counter() - t0
-    for g in insts:
-        g.free()
+    for j in live:
+        insts[j].free()
     b.free()
+    lu = float(sum(insts[j].n * steps_of[j] for j in range(len(insts))))
     if log:
-        log("sweep: batched %d sites x %d steps in %.3fs" % (total, steps, t_batched))
+        log("sweep: batched %d sites x %d steps in %.3fs, %d instances retired non-finite"
+            % (total, steps, t_batched, len(nonfinite)))
 
-    # sequential: the same instances one at a time through a one-instance engine
+    # sequential: the same instances one at a time through a one-instance
+    # engine, each for the steps it ran in the batch
     t_seq = 0.0
-    for solid, prm, c0 in problems:
+    for j, (solid, prm, c0) in enumerate(problems):
         g = MultiPhaseGPU(precision).init(solid, prm, sw)
         g.init_rest(c0, 1.0)
         g.record_target_mass()
         g.step(2)
         g.batch.synchronize()
         t0 = time.perf_counter()
-        g.step(steps)
+        g.step(steps_of[j])
         g.batch.synchronize()
         t_seq += time.perf_counter() - t0
         g.free()
         g.batch.free()
     if log:
         log("sweep: sequential %.3fs" % t_seq)
-    lu = float(total) * steps
     res = {"instances": len(problems), "sites": int(total), "steps": steps,
            "batched_MLUPS": lu / t_batched / 1e6, "sequential_MLUPS": lu / t_seq / 1e6,
            "speedup": t_seq / t_batched, "batched_s": t_batched, "sequential_s": t_seq,
+           "lattice_updates": lu, "retired_nonfinite": len(nonfinite),
+           "retired": [{"instance": j, "after_steps": steps_of[j], "theta": specs[j]["theta"],
+                        "mobility_coeff": specs[j]["mobility_coeff"]} for j in nonfinite],
            "autotune": tune}
     if check_fp64 and precision == "f32":
         f32 = _batch_fields(problems, "f32", steps, graph_chunk)
@@ -198,6 +234,8 @@ class GpuBackend:
         self.batch.step(nsteps, stats_last=True)
 
     def mean_c(self, g):
+        """Mean c over the last step; raises FelbmError(FELBM_ENONFINITE) when
+        the instance's fields are no longer finite."""
         return float(g.flow_stats()[4]) / g.n
 
     def retire(self, g):
@@ -238,7 +276,9 @@ class Server:
             # admission (FIFO, budget permitting)
             while pending and pending[0]["arrival"] <= now:
                 r = pending[0]
-                need = instance_bytes(int(np.count_nonzero(problems[r["id"]]["solid"] == 0)), self.precision)
+                pr = problems[r["id"]]
+                n_open = sum(len(pr.get(k, ())) for k in ("inlet", "outlet", "buf_verts"))
+                need = instance_bytes(int(np.count_nonzero(pr["solid"] == 0)), self.precision, n_open)
                 if used + need > self.budget and active:
                     break
                 pending.pop(0)
@@ -255,28 +295,42 @@ class Server:
             still = []
             for a in active:
                 a["steps"] += self.check_every
-                m = self.be.mean_c(a["h"])
-                conv = a["last"] is not None and abs(m - a["last"]) <= self.tol * max(abs(m), 1e-300)
+                try:
+                    m = self.be.mean_c(a["h"])
+                except FelbmError as err:
+                    if err.code != FELBM_ENONFINITE:
+                        raise
+                    m = float("nan")
+                finite = m == m
+                rel = abs(m - a["last"]) / max(abs(m), 1e-300) if (finite and a["last"] is not None) else None
+                conv = rel is not None and rel <= self.tol
                 a["last"] = m
-                if conv or a["steps"] >= self.max_steps:
+                if conv or not finite or a["steps"] >= self.max_steps:
                     t = clock()
                     self.be.retire(a["h"])
                     used -= a["bytes"]
+                    status = "converged" if conv else ("nonfinite" if not finite else "step cap")
                     done.append({"id": a["req"]["id"], "arrival": a["req"]["arrival"], "done": t,
                                  "latency": t - a["req"]["arrival"], "steps": a["steps"], "sites": a["n"],
-                                 "converged": bool(conv), "mean_c": m})
+                                 "converged": bool(conv), "status": status, "mean_c": m, "rel_change": rel})
                     if log:
-                        log("serve: request %d done after %d steps (%s), latency %.2fs"
-                            % (a["req"]["id"], a["steps"], "converged" if conv else "step cap", t - a["req"]["arrival"]))
+                        log("serve: request %d done after %d steps (%s, last relative change of mean c %s), "
+                            "latency %.2fs" % (a["req"]["id"], a["steps"], status,
+                                               "%.2e" % rel if rel is not None else "-", t - a["req"]["arrival"]))
                 else:
                     still.append(a)
             active = still
         makespan = max(d["done"] for d in done) - min(d["arrival"] for d in done) if done else 0.0
         lat = np.array([d["latency"] for d in done])
+        rels = [d["rel_change"] for d in done if d["rel_change"] is not None]
         return {"requests": len(done), "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
                 "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
                 "aggregate_MLUPS": lu / makespan / 1e6 if makespan > 0 else None,
                 "makespan_s": makespan, "converged": int(sum(d["converged"] for d in done)),
+                "nonfinite": int(sum(d["status"] == "nonfinite" for d in done)),
+                "step_cap": int(sum(d["status"] == "step cap" for d in done)),
+                "rel_change_at_retirement": {"min": min(rels), "median": float(np.median(rels)), "max": max(rels)}
+                if rels else None,
                 "lattice_updates": lu, "done": done}
 
 

@@ -177,13 +177,14 @@ def workload(name):
     raise KeyError(name)
 
 
-def sweep_instances(count=48, seed=1000):
+def sweep_instances(count=48, seed=1000, edges=(64, 96, 128, 160)):
     """The parameter sweep: per-instance edge, porosity, contact angle and
-    mobility drawn from a seeded stream; instance i uses geometry seed+i."""
+    mobility drawn from a seeded stream; instance i uses geometry seed+i.
+    edges=(64,) gives the many-small-instances form of the same sweep."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
-        edge = int(rng.choice([64, 96, 128, 160]))
+        edge = int(rng.choice(list(edges)))
         por = float(rng.uniform(0.30, 0.50))
         theta = float(rng.uniform(30.0, 150.0))
         mob = float(math.exp(rng.uniform(math.log(0.005), math.log(0.05))))

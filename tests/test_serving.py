@@ -108,3 +108,40 @@ def test_admission_is_fifo_and_respects_the_hbm_budget():
         peak = max(peak, len(live))
     assert peak == 3
     assert order == sorted(order)
+
+
+class BlowUpBackend(FakeBackend):
+    """request `bad` raises the engine's non-finite error from its third check on."""
+
+    def __init__(self, clock, rates, sizes, bad):
+        super().__init__(clock, rates, sizes)
+        self.bad = bad
+
+    def mean_c(self, rid):
+        if rid == self.bad and self.active[rid] >= 3:
+            raise S.FelbmError(S.FELBM_ENONFINITE, "non-finite or out-of-range value in the monitored fields")
+        return super().mean_c(rid)
+
+
+def test_nonfinite_request_is_retired_and_frees_its_budget():
+    n = 6
+    sizes = {i: 400000 for i in range(n)}
+    per = S.instance_bytes(400000, "f32")
+    clk = FakeClock()
+    be = BlowUpBackend(clk, {i: 0.999999 for i in range(n)}, sizes, bad=1)
+    reqs, probs = make(n, sizes)
+    res = S.Server(be, hbm_budget=2.5 * per, check_every=500, tol=1e-6, max_steps=4000, clock=clk,
+                   sleep=clk.sleep).run(reqs, probs)
+    by_id = {d["id"]: d for d in res["done"]}
+    assert res["requests"] == n and res["nonfinite"] == 1 and res["step_cap"] == n - 1
+    assert by_id[1]["status"] == "nonfinite" and by_id[1]["steps"] == 1500 and not by_id[1]["converged"]
+    assert all(by_id[i]["status"] == "step cap" and by_id[i]["steps"] == 4000 for i in range(n) if i != 1)
+    # the retired request's slot went to the next one in line at once
+    admit_t = {rid: t for ev, rid, t in be.log if ev == "admit"}
+    retire_t = {rid: t for ev, rid, t in be.log if ev == "retire"}
+    assert admit_t[2] == retire_t[1]
+
+
+def test_open_boundary_allocation_is_in_the_budget():
+    assert S.instance_bytes(10 ** 6, "f32", n_open=20000) - S.instance_bytes(10 ** 6, "f32") == \
+        4 * 10 ** 6 + 8 * 4 * 20000 + 512
