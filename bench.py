@@ -26,11 +26,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # algorithmic values per site per launch (FP32 -> x4 bytes): moments reads h and g
 # (38) and writes c_raw, p0, j (5); the c pass reads c_raw, p0, j (5) and writes
-# c, mu, p* (3); the mu pass reads mu (1) and packs c, p*, lapmu, mu into one
-# record per site (+2 read, +4 written); the collision reads g (19) and the own
-# record (4) and writes h and g (38).  Neighbour values come from cache-resident
+# c, mu, p* (3); the mu pass reads mu (1) and packs c, p*, lapmu into one record
+# per site (+2 read, +3 written); the collision reads g (19), the own record (3)
+# and mu (1) and writes h and g (38).  Neighbour values come from cache-resident
 # lines and are not counted.
-ALG_VALUES = {"moments": 43, "cfield": 8, "mufield": 7, "collide": 61}
+ALG_VALUES = {"moments": 43, "cfield": 8, "mufield": 6, "collide": 61}
 B_ALG_PER_LU = 452  # compulsory bytes per lattice update, FP32 (113 values)
 
 

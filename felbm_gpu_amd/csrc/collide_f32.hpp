@@ -95,7 +95,8 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       }
     } else {
       float cN[Q], pN[Q], c2[Q], p2[Q], lN[Q];
-      const Rec4<float> r0 = bld_rec4((const float *)nullptr, rF, io, F_REC*nsb);
+      const Rec4<float> r0 = bld_r3(rF, io, F_REC*nsb);
+      const float mu_own = bld_nt<float>(rF, io, F_MU*nsb);
       const float craw = bld_nt<float>(rF, io, F_CRAW*nsb);      // read once
       const float clo = bld_nt<float>(rF, io, F_CLO*nsb);        // Sigma h - c_raw (moments pass)
       {
@@ -104,14 +105,14 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
         { unsigned so2; ring_f<float, OPEN, 2, false>(d, rW, nb, io, vo2, fl2, so2); }
         #pragma unroll
         for (int m = 1; m < Q; ++m) {
-          const Rec4<float> r1 = bld_rec3(rF, vo[m], F_REC*nsb);
-          const Rec2<float> r2 = bld_rec2((const float *)nullptr, rF, vo2[m], F_REC*nsb);
+          const Rec4<float> r1 = bld_r3(rF, vo[m], F_REC*nsb);
+          const Rec2<float> r2 = bld_r2(rF, vo2[m], F_REC*nsb);
           const bool s1 = (fl >> m) & 1u, s2 = (fl2 >> m) & 1u;
           cN[m] = s1 ? r1.c : r0.c; pN[m] = s1 ? r1.p : r0.p; lN[m] = s1 ? r1.l : r0.l;
           c2[m] = s2 ? r2.c : r0.c; p2[m] = s2 ? r2.p : r0.p;
         }
       }
-      const float c = r0.c, p = r0.p, mu = r0.m, lmu = r0.l;
+      const float c = r0.c, p = r0.p, mu = mu_own, lmu = r0.l;
       // The mass correction c = c_raw - lambda phi+ (applied by the c pass) is
       // an FP32 value, and near the interface the correction per step (1e-7 to
       // 1e-6) is carried only to half an ulp of c (3e-8 at c = 0.5): the same

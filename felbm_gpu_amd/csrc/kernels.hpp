@@ -233,17 +233,18 @@ __device__ __forceinline__ void bst(double v, rsrc_t r, unsigned vo, unsigned so
 // site (4 ns); [c, p*] interleaved per site (2 ns); mu; the FP32 remainder
 // Sigma h - c_raw of the moments pass (unused in FP64); then 4 x ns
 // holding either what the mu-stencil pass writes for the collision -- FP32:
-// the per-site records [c, p*, lapmu, mu] (one vector load per neighbour);
-// FP64: lapmu alone (F_LMU), the collision reading the [c, p*] pairs, whose 16
-// bytes a record would double -- or, between steps, the exported rho, ux, uy,
-// uz of a field download.  A pass reads its own
-// site's inputs with one vector access per group: every vector memory
-// instruction costs the address unit the same whatever its width.
+// 12-byte per-site records [c, p*, lapmu] (one vector load per neighbour; the
+// collision reads mu only at its own site, from its field, so the records the
+// +-2 planes keep in L2 are 12 rather than 16 bytes a site: collision 0.725 ->
+// 0.699 ms, mu pass 0.135 -> 0.129 ms at 300^3); FP64: lapmu alone (F_LMU),
+// the collision reading the [c, p*] pairs, whose 16 bytes a record would
+// double -- or, between steps, the exported rho, ux, uy, uz of a field
+// download.  A pass reads its own site's inputs with one vector access per
+// group.
 enum { F_CRAW = 0, F_PJ = 1, F_CP = 5, F_MU = 7, F_CLO = 8, F_REC = 9, F_LMU = 9, F_COUNT = 13 };
 
-// record loads/stores: (c, p*) or (c, p*, lapmu, mu) of one site, vo = site
-// byte offset in a scalar field (i * sizeof(real)); the record sits at 4*vo
-// The loaded vector is bit-cast as a whole: a bit-cast of one element of the
+// record loads/stores of one site, vo = site byte offset in a scalar field
+// (i * sizeof(real)).  The loaded vector is bit-cast as a whole: a bit-cast of one element of the
 // builtin's result (bit_cast<float>(v[1])) compiles to a read of element 0 with
 // this toolchain, which narrows the load to a single dword.
 template <typename real> struct Rec2 { real c, p; };
@@ -314,30 +315,23 @@ __device__ __forceinline__ Rec2<double> bld_cp(const double *, rsrc_t r, unsigne
 }
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ Rec2<float> bld_rec2(const float *, rsrc_t r, unsigned vo, unsigned so)
-{
-  const f32x2 f = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(vo*4u), (int)so, 0));
-  return { f[0], f[1] };
-}
-// c, p*, lap(mu) of a record (its first 12 bytes)
-__device__ __forceinline__ Rec4<float> bld_rec3(rsrc_t r, unsigned vo, unsigned so)
+// 12-byte records [c, p*, lapmu] at 12*i (vo = 4*i)
+__device__ __forceinline__ Rec4<float> bld_r3(rsrc_t r, unsigned vo, unsigned so)
 {
   typedef float f32x3 __attribute__((ext_vector_type(3)));
-  const f32x3 f = __builtin_bit_cast(f32x3, __builtin_amdgcn_raw_buffer_load_b96(r, (int)(vo*4u), (int)so, 0));
+  const f32x3 f = __builtin_bit_cast(f32x3, __builtin_amdgcn_raw_buffer_load_b96(r, (int)(vo*3u), (int)so, 0));
   return { f[0], f[1], f[2], 0.0f };
 }
-__device__ __forceinline__ Rec4<float> bld_rec4(const float *, rsrc_t r, unsigned vo, unsigned so)
+__device__ __forceinline__ Rec2<float> bld_r2(rsrc_t r, unsigned vo, unsigned so)
 {
-  const f32x4 f = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(vo*4u), (int)so, 0));
-  return { f[0], f[1], f[2], f[3] };
+  const f32x2 f = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(vo*3u), (int)so, 0));
+  return { f[0], f[1] };
 }
-template <int AUX = 0>
-__device__ __forceinline__ void bst_rec4(float c, float p, float l, float m, rsrc_t r, unsigned vo, unsigned so)
+__device__ __forceinline__ void bst_r3(float c, float p, float l, rsrc_t r, unsigned vo, unsigned so)
 {
-  typedef __attribute__((ext_vector_type(4))) unsigned u4;
-  u4 v = { __builtin_bit_cast(unsigned, c), __builtin_bit_cast(unsigned, p),
-           __builtin_bit_cast(unsigned, l), __builtin_bit_cast(unsigned, m) };
-  __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(vo*4u), (int)so, AUX);
+  typedef __attribute__((ext_vector_type(3))) unsigned u3;
+  u3 v = { __builtin_bit_cast(unsigned, c), __builtin_bit_cast(unsigned, p), __builtin_bit_cast(unsigned, l) };
+  __builtin_amdgcn_raw_buffer_store_b96(v, r, (int)(vo*3u), (int)so, 0);
 }
 
 // ---- lattice lookup ---------------------------------------------------------------
@@ -964,7 +958,7 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
     // own [c, p*], read once: non-temporal, like the other single-use streams,
     // so L2 keeps the stencil data (mu here, the records in the collision pass)
     const Rec2<real> cp = bld_cp<2>((const real *)nullptr, rF, io, F_CP*nsb);
-    bst_rec4(cp.c, cp.p, s, m0, rF, io, F_REC*nsb);
+    bst_r3(cp.c, cp.p, s, rF, io, F_REC*nsb);
   } else {
     // FP64: lapmu alone; the collision takes c and p* from the [c, p*] pairs
     // and mu from its field (one 16-byte and one 8-byte load per ring-1
