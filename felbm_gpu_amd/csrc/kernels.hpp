@@ -416,11 +416,15 @@ __device__ __forceinline__ unsigned nb_lookup(const Inst<real> &d, rsrc_t rW, co
   return fluid ? idx*(unsigned)sizeof(real) : self_off;
 }
 
-// One ring (S = 1: x + e_m, S = 2: x + 2 e_m).  Each of its 9 rows is read
-// once as a window; the wrapped window of each row that can need it (dy == 0
-// or dz == 0, entered off x) is read in the same round trip, in waves that
-// have a lane within 2 of an x face.  Reading the wrapped words only after
-// the local ones came back (a second round trip in half the waves) cost more
+// One ring (S = 1: x + e_m, S = 2: x + 2 e_m).  The rows entered off x
+// (dy == 0 or dz == 0) are read as a window of the two words that hold
+// x-2..x+2; only one position of the ring can wrap round the x axis (x - S
+// when x < S, x + S when x + S >= Nx), so those rows add the one word of that
+// position, read in the same round trip in waves that have a lane within S
+// of an x face.  The diagonal rows (dy, dz both nonzero) hold only x itself:
+// one word.  (Two-word windows for the wrapped part and for the diagonal
+// rows held 18 more registers.)  Reading the wrapped words only after the
+// local ones came back (a second round trip in half the waves) cost more
 // than it saved.
 // SELF = false: a neighbour that is not a site gets the offset of the next
 // site along its row instead of the own offset, so a wave's load for one
@@ -431,26 +435,33 @@ template <typename real, bool OPEN, int S, bool SELF = true>
 __device__ __forceinline__ void ring_f(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
                                        unsigned (&vo)[Q], unsigned &fl, unsigned &so)
 {
+  (void)d;
   const int x2 = nb.xw[2];
-  const unsigned xf = x2 < 2 ? (unsigned)(d.Nx >= 2 ? d.Nx - 2 : 0) >> 5 : 0u;
-  RowWin L[3][3], F[3][3];
+  const unsigned wf = (unsigned)(x2 < S ? nb.xw[2 - S] : nb.xw[2 + S]) >> 5, w0 = (unsigned)x2 >> 5;
+  RowWin L[3][3];
+  uint64_t F[3][3];
   #pragma unroll
   for (int a = 0; a < 3; ++a)
     #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const unsigned row = (unsigned)nb.zr[2 + S*(c - 1)] + (unsigned)nb.yr[2 + S*(a - 1)];
-      L[a][c] = row_win(rW, row, nb.xb);
-      F[a][c] = RowWin{0ull, 0ull};
-      if ((a == 1 || c == 1) && nb.anyw[S - 1]) F[a][c] = row_win(rW, row, xf);
+      if (a == 1 || c == 1) {
+        L[a][c] = row_win(rW, row, nb.xb);
+        F[a][c] = nb.anyw[S - 1] ? bld<uint64_t>(rW, (row + wf)*8u, 0u) : 0ull;
+      } else {
+        L[a][c] = RowWin{0ull, 0ull};
+        F[a][c] = bld<uint64_t>(rW, (row + w0)*8u, 0u);
+      }
     }
   vo[0] = self_off; fl = S == 1 ? 1u : 0u; so = 0u;
   #pragma unroll
   for (int m = 1; m < Q; ++m) {
     const int dx = S*CX[m], dy = S*CY[m], dz = S*CZ[m];
-    const RowWin &Lw = L[CY[m] + 1][CZ[m] + 1], &Fw = F[CY[m] + 1][CZ[m] + 1];
+    const RowWin &Lw = L[CY[m] + 1][CZ[m] + 1];
+    const uint64_t Fw = F[CY[m] + 1][CZ[m] + 1];
     const int xx = nb.xw[dx + 2];
     const unsigned wi = (unsigned)xx >> 5;
-    const uint64_t w = wi == nb.xb ? Lw.a : (wi == nb.xb + 1u ? Lw.b : (wi == xf ? Fw.a : Fw.b));
+    const uint64_t w = (CY[m] != 0 && CZ[m] != 0) ? Fw : (wi == nb.xb ? Lw.a : (wi == nb.xb + 1u ? Lw.b : Fw));
     const unsigned lo = (unsigned)w, b = (unsigned)xx & 31u;
     const unsigned idx = (unsigned)(w >> 32) + (unsigned)__popc(lo & ((1u << b) - 1u));
     bool f = (lo >> b) & 1u, inbox = true;
