@@ -220,13 +220,18 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
       float rr[Q], yv[Q];
       float relx = 0.0f;
       if constexpr (!MRT) relx = relax;
+      float ukp = 0, fpp = 0;        // e_k.u and e_k.bf of the odd member of the pair
       #pragma unroll
       for (int k = 0; k < Q; ++k) {
         const float w = W<float>(k);
-        float uk = 0, fp = 0;
-        if (CX[k] > 0) { uk += ux; fp += bfx; } else if (CX[k] < 0) { uk -= ux; fp -= bfx; }
-        if (CY[k] > 0) { uk += uy; fp += bfy; } else if (CY[k] < 0) { uk -= uy; fp -= bfy; }
-        if (CZ[k] > 0) { uk += uz; fp += bfz; } else if (CZ[k] < 0) { uk -= uz; fp -= bfz; }
+        // the even member k = opp(k - 1) takes the negated projections (exact)
+        if (k == 0 || (k & 1)) {
+          ukp = 0; fpp = 0;
+          if (CX[k] > 0) { ukp += ux; fpp += bfx; } else if (CX[k] < 0) { ukp -= ux; fpp -= bfx; }
+          if (CY[k] > 0) { ukp += uy; fpp += bfy; } else if (CY[k] < 0) { ukp -= uy; fpp -= bfy; }
+          if (CZ[k] > 0) { ukp += uz; fpp += bfz; } else if (CZ[k] < 0) { ukp -= uz; fpp -= bfz; }
+        }
+        const float uk = (k == 0 || (k & 1)) ? ukp : -ukp, fp = (k == 0 || (k & 1)) ? fpp : -fpp;
         const float u_term = uk*(3.0f + 4.5f*uk) - gamma_u_sq;
         const float wut = w*u_term;
         const float G = w + wut;

@@ -464,14 +464,19 @@ __device__ __forceinline__ void ring_f(const Inst<real> &d, rsrc_t rW, const Nbh
     const uint64_t w = (CY[m] != 0 && CZ[m] != 0) ? Fw : (wi == nb.xb ? Lw.a : (wi == nb.xb + 1u ? Lw.b : Fw));
     const unsigned lo = (unsigned)w, b = (unsigned)xx & 31u;
     const unsigned idx = (unsigned)(w >> 32) + (unsigned)__popc(lo & ((1u << b) - 1u));
-    bool f = (lo >> b) & 1u, inbox = true;
+    // the flag words are built with integer bit operations (a bool per
+    // direction became a lane mask held in scalar registers, 36 of them
+    // across the two rings of the collision pass)
+    const unsigned fb = (lo >> b) & 1u;
+    unsigned ib = 1u;
     if constexpr (OPEN) {
       const unsigned bits = (1u << (dx + 2)) | (1u << (7 + dy)) | (1u << (12 + dz));
-      inbox = (nb.ok & bits) == bits;
+      ib = (nb.ok & bits) == bits ? 1u : 0u;
     }
-    const bool fluid = f && inbox, solid = !f && inbox;
-    vo[m] = (fluid || !SELF) ? idx*(unsigned)sizeof(real) : self_off;
-    fl |= fluid ? (1u << m) : 0u; so |= solid ? (1u << m) : 0u;
+    const unsigned fluid = fb & ib;
+    if constexpr (SELF) vo[m] = fluid ? idx*(unsigned)sizeof(real) : self_off;
+    else vo[m] = idx*(unsigned)sizeof(real);
+    fl |= fluid << m; so |= ((fb ^ 1u) & ib) << m;
   }
 }
 
