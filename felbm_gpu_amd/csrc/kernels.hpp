@@ -416,43 +416,111 @@ __device__ __forceinline__ unsigned nb_lookup(const Inst<real> &d, rsrc_t rW, co
   return fluid ? idx*(unsigned)sizeof(real) : self_off;
 }
 
-// One ring (S = 1: x + e_m, S = 2: x + 2 e_m).  The rows entered off x
-// (dy == 0 or dz == 0) are read as a window of the two words that hold
-// x-2..x+2; only one position of the ring can wrap round the x axis (x - S
-// when x < S, x + S when x + S >= Nx), so those rows add the one word of that
-// position, read in the same round trip in waves that have a lane within S
-// of an x face.  The diagonal rows (dy, dz both nonzero) hold only x itself:
-// one word.  (Two-word windows for the wrapped part and for the diagonal
-// rows held 18 more registers.)  Reading the wrapped words only after the
-// local ones came back (a second round trip in half the waves) cost more
-// than it saved.
-// SELF = false: a neighbour that is not a site gets the offset of the next
-// site along its row instead of the own offset, so a wave's load for one
-// direction stays within the lines of that row (the caller substitutes the
-// own value); the own offset would add the lines of the own row to every
-// such load.
-template <typename real, bool OPEN, int S, bool SELF = true>
-__device__ __forceinline__ void ring_f(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
-                                       unsigned (&vo)[Q], unsigned &fl, unsigned &so)
+// The words of the 9 neighbour rows of ring S: a two-word window for the rows
+// entered off x, plus the word of the one position that can wrap round x;
+// the word of x itself for the diagonal rows.
+struct RingWords { RowWin L[3][3]; uint64_t F[3][3]; };
+template <typename real, int S>
+__device__ __forceinline__ void ring_words_global(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, RingWords &rw)
 {
   (void)d;
   const int x2 = nb.xw[2];
   const unsigned wf = (unsigned)(x2 < S ? nb.xw[2 - S] : nb.xw[2 + S]) >> 5, w0 = (unsigned)x2 >> 5;
-  RowWin L[3][3];
-  uint64_t F[3][3];
   #pragma unroll
   for (int a = 0; a < 3; ++a)
     #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const unsigned row = (unsigned)nb.zr[2 + S*(c - 1)] + (unsigned)nb.yr[2 + S*(a - 1)];
       if (a == 1 || c == 1) {
-        L[a][c] = row_win(rW, row, nb.xb);
-        F[a][c] = nb.anyw[S - 1] ? bld<uint64_t>(rW, (row + wf)*8u, 0u) : 0ull;
+        rw.L[a][c] = row_win(rW, row, nb.xb);
+        rw.F[a][c] = nb.anyw[S - 1] ? bld<uint64_t>(rW, (row + wf)*8u, 0u) : 0ull;
       } else {
-        L[a][c] = RowWin{0ull, 0ull};
-        F[a][c] = bld<uint64_t>(rW, (row + w0)*8u, 0u);
+        rw.L[a][c] = RowWin{0ull, 0ull};
+        rw.F[a][c] = bld<uint64_t>(rW, (row + w0)*8u, 0u);
       }
     }
+}
+
+// ---- workgroup row cache --------------------------------------------------------
+// A workgroup's sites are consecutive in the x-fastest site list.  When they
+// lie in one z plane, in rows y0..y1, every ring-S neighbour row of each of
+// them is one of (y0-S .. y1+S) x (z0-S, z0, z0+S): the workgroup reads those
+// rows whole into LDS, a few coalesced loads per wave, and each lane takes its
+// neighbour words from there instead of reading a window per row from the
+// table (9 to 14 vector memory instructions per ring and wave: the passes are
+// bound by their vector memory instructions).  A workgroup that spans two
+// planes, or whose rows do not fit, reads the table.  The moments and c
+// passes use it (0.414 -> 0.396 and 0.148 -> 0.145 ms at 300^3); in the mu
+// pass it gained nothing, and in the collision, at 3 workgroups per CU, the
+// fill and barrier in front of every wave cost more than the windows
+// (0.662 -> 0.666 ms with ring 1 cached, 0.708 with both rings).
+constexpr int RC_WORDS = 512;                  // per ring: 4 KiB of LDS
+struct RowCache { int y0, nr, z0, on1, on2; };
+
+// the plan, from the first and last site of the workgroup (uniform loads)
+template <typename real>
+__device__ __forceinline__ RowCache rc_plan(const Inst<real> &d, int first)
+{
+  const int last = min(first + (int)blockDim.x - 1, d.n - 1);
+  int x0, y0, z0, x1, y1, z1;
+  decode(d, first, x0, y0, z0);
+  decode(d, last, x1, y1, z1);
+  RowCache rc;
+  rc.y0 = y0; rc.z0 = z0; rc.nr = y1 - y0 + 1;
+  rc.on1 = z0 == z1 && 3*(rc.nr + 2)*d.wpr + 1 <= RC_WORDS;
+  rc.on2 = z0 == z1 && 3*(rc.nr + 4)*d.wpr + 1 <= RC_WORDS;
+  return rc;
+}
+
+// rows (y0-S+j, z0+S(c-1)), j < nr+2S, c < 3, each wpr words, in that order;
+// the caller synchronises the workgroup before reading them
+template <typename real, int S>
+__device__ __forceinline__ void rc_fill(const Inst<real> &d, const RowCache &rc, uint64_t *lds)
+{
+  const int wpr = d.wpr, nrs = rc.nr + 2*S, total = 3*nrs*wpr;
+  const float inv = 1.0f/(float)wpr;
+  for (int e = (int)threadIdx.x; e < total; e += (int)blockDim.x) {
+    int q = (int)((float)e*inv), r = e - q*wpr;           // e = q*wpr + r, q = row slot
+    if (r < 0) { --q; r += wpr; } else if (r >= wpr) { ++q; r -= wpr; }
+    const int c = q >= 2*nrs ? 2 : (q >= nrs ? 1 : 0), j = q - c*nrs;
+    bool ok;
+    const int y = wrap_axis(rc.y0 - S + j, d.Ny, 1, ok), z = wrap_axis(rc.z0 + S*(c - 1), d.Nz, 1, ok);
+    lds[e] = ldg(d.words, ((long)z*d.Ny + y)*wpr + r);
+  }
+}
+
+// The neighbour words of ring S from the workgroup's row cache (rc_fill):
+// the same words ring_words_global reads from the table.  ly = the lane's row
+// y minus the cache's first own row.
+template <int S>
+__device__ __forceinline__ void ring_words_lds(const uint64_t *lds, const RowCache &rc, int wpr, int ly,
+                                               const Nbhd &nb, RingWords &rw)
+{
+  const int x2 = nb.xw[2];
+  const unsigned wf = (unsigned)(x2 < S ? nb.xw[2 - S] : nb.xw[2 + S]) >> 5, w0 = (unsigned)x2 >> 5;
+  const int nrs = rc.nr + 2*S;
+  #pragma unroll
+  for (int a = 0; a < 3; ++a)
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const unsigned base = (unsigned)((c*nrs + ly + S*a)*wpr);
+      if (a == 1 || c == 1) {
+        rw.L[a][c] = RowWin{lds[base + nb.xb], lds[base + nb.xb + 1u]};
+        rw.F[a][c] = nb.anyw[S - 1] ? lds[base + wf] : 0ull;
+      } else {
+        rw.L[a][c] = RowWin{0ull, 0ull};
+        rw.F[a][c] = lds[base + w0];
+      }
+    }
+}
+
+// byte offsets, fluid and solid bits of ring S from its words
+template <typename real, bool OPEN, int S, bool SELF>
+__device__ __forceinline__ void ring_offsets(const Nbhd &nb, const RingWords &rw, unsigned self_off,
+                                             unsigned (&vo)[Q], unsigned &fl, unsigned &so)
+{
+  const auto &L = rw.L;
+  const auto &F = rw.F;
   vo[0] = self_off; fl = S == 1 ? 1u : 0u; so = 0u;
   #pragma unroll
   for (int m = 1; m < Q; ++m) {
@@ -478,6 +546,30 @@ __device__ __forceinline__ void ring_f(const Inst<real> &d, rsrc_t rW, const Nbh
     else vo[m] = idx*(unsigned)sizeof(real);
     fl |= fluid << m; so |= ((fb ^ 1u) & ib) << m;
   }
+}
+
+// One ring (S = 1: x + e_m, S = 2: x + 2 e_m).  The rows entered off x
+// (dy == 0 or dz == 0) are read as a window of the two words that hold
+// x-2..x+2; only one position of the ring can wrap round the x axis (x - S
+// when x < S, x + S when x + S >= Nx), so those rows add the one word of that
+// position, read in the same round trip in waves that have a lane within S
+// of an x face.  The diagonal rows (dy, dz both nonzero) hold only x itself:
+// one word.  (Two-word windows for the wrapped part and for the diagonal
+// rows held 18 more registers.)  Reading the wrapped words only after the
+// local ones came back (a second round trip in half the waves) cost more
+// than it saved.
+// SELF = false: a neighbour that is not a site gets the offset of the next
+// site along its row instead of the own offset, so a wave's load for one
+// direction stays within the lines of that row (the caller substitutes the
+// own value); the own offset would add the lines of the own row to every
+// such load.
+template <typename real, bool OPEN, int S, bool SELF = true>
+__device__ __forceinline__ void ring_f(const Inst<real> &d, rsrc_t rW, const Nbhd &nb, unsigned self_off,
+                                       unsigned (&vo)[Q], unsigned &fl, unsigned &so)
+{
+  RingWords rw;
+  ring_words_global<real, S>(d, rW, nb, rw);
+  ring_offsets<real, OPEN, S, SELF>(nb, rw, self_off, vo, fl, so);
 }
 
 // r = 1 ring: byte offsets vo[m] of x + e_m (own offset when not a site),
@@ -636,6 +728,10 @@ __global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ 
     unsigned long long *o = d.acc + ACC_MASS + (cur ^ 1u)*5*NSLOT;
     for (int q = threadIdx.x; q < 5*NSLOT; q += blockDim.x) stg(o, q, 0ull);
   }
+  if (bm.y >= d.n) return;                          // a padding workgroup: no sites
+  __shared__ uint64_t rcw[RC_WORDS];
+  const RowCache rc = rc_plan(d, bm.y);
+  if (rc.on1) { rc_fill<real, 1>(d, rc, rcw); __syncthreads(); }
   Fx m = {0, 0, 0}, w = {0, 0, 0};
   if (i < d.n) {
     const unsigned io = (unsigned)i*RB;
@@ -643,7 +739,12 @@ __global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ 
     int x, y, z; decode(d, i, x, y, z);
     Nbhd nb; make_nbhd(d, x, y, z, nb);
     unsigned vo[Q], fl, so;
-    ring1<real, OPEN>(d, rW, nb, io, vo, fl, so);
+    {
+      RingWords rw;
+      if (rc.on1) ring_words_lds<1>(rcw, rc, d.wpr, y - rc.y0, nb, rw);
+      else ring_words_global<real, 1>(d, rW, nb, rw);
+      ring_offsets<real, OPEN, 1, true>(nb, rw, io, vo, fl, so);
+    }
     (void)so;
     // c accumulates in FP64 (the FP64 state sums alike).  The FP32 field c
     // keeps Sigma h to half an ulp (6e-8 near c = 1), below which the slow
@@ -840,6 +941,10 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
     apply = d.lam_on != 0;
     lambda = d.lam;
   }
+  if (bm.y >= d.n) return;                          // a padding workgroup: no sites
+  __shared__ uint64_t rcw[RC_WORDS];
+  const RowCache rc = rc_plan(d, bm.y);
+  if (rc.on1) { rc_fill<real, 1>(d, rc, rcw); __syncthreads(); }
   const int i = bm.y + (int)threadIdx.x;
   if (i >= d.n) return;
   constexpr unsigned RB = sizeof(real);
@@ -848,7 +953,12 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
   int x, y, z; decode(d, i, x, y, z);
   Nbhd nb; make_nbhd(d, x, y, z, nb);
   unsigned vo[Q], fl, so;
-  ring_f<real, OPEN, 1, false>(d, rW, nb, io, vo, fl, so);
+  {
+    RingWords rw;
+    if (rc.on1) ring_words_lds<1>(rcw, rc, d.wpr, y - rc.y0, nb, rw);
+    else ring_words_global<real, 1>(d, rW, nb, rw);
+    ring_offsets<real, OPEN, 1, false>(nb, rw, io, vo, fl, so);
+  }
   const real craw = bld<real>(rF, io, F_CRAW*nsb);
   const real c = corrected<real>(craw, apply, lambda);
   real cn[Q];
@@ -943,6 +1053,7 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
   const int2 bm = sdi >= 0 ? make_int2(sdi, logical_block()*(int)blockDim.x) : map[logical_block()];
   if (bm.x < 0) return;
   const Inst<real> &d = D[bm.x];
+  if (bm.y >= d.n) return;                          // a padding workgroup: no sites
   const int i = bm.y + (int)threadIdx.x;
   if (i >= d.n) return;
   constexpr unsigned RB = sizeof(real);
