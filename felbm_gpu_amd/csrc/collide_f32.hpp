@@ -28,10 +28,8 @@ constexpr bool CONS_ROW[19] = { true, false, false, true, false, true, false, tr
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 template <bool MRT, bool HBB, bool OPEN>
-__global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restrict__ D,
-                                                     const int2 *__restrict__ map, int sdi, unsigned T, int stats)
+__device__ __forceinline__ void collide_f32_body(const Inst<float> *__restrict__ D, const int2 bm, unsigned T, int stats)
 {
-  const int2 bm = sdi >= 0 ? make_int2(sdi, logical_block()*(int)blockDim.x) : map[logical_block()];
   if (bm.x < 0) return;
   const Inst<float> &d = D[bm.x];
   const unsigned cur = (T ^ (unsigned)d.par0) & 1u;
@@ -301,6 +299,17 @@ __global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restri
                                  (unsigned long long)s4.hi, s4.lo, s0.bad + s3.bad + s4.bad, vmax };
     block_accumulate<12>(v, d.acc + ACC_STATS, NSLOT, (int)(blockIdx.x & (NSLOT - 1)), true);
   }
+}
+
+template <bool MRT, bool HBB, bool OPEN>
+__global__ void __launch_bounds__(256) k_collide_f32(const Inst<float> *__restrict__ D,
+                                                     const int2 *__restrict__ map, int sdi, unsigned T, int stats)
+{
+  const int2 bm = map_entry(map, sdi);
+  if constexpr (OPEN) {
+    if (bm.x >= 0 && (bm.x & MAP_INTERIOR)) { collide_f32_body<MRT, HBB, false>(D, make_int2(bm.x & ~MAP_INTERIOR, bm.y), T, stats); return; }
+  }
+  collide_f32_body<MRT, HBB, OPEN>(D, bm, T, stats);
 }
 
 }  // namespace fm

@@ -34,6 +34,9 @@ struct Slot {
   int Nx = 0, Ny = 0, Nz = 0, wpr = 0;
   int per[3] = {1, 1, 1};
   int hbb = 1, mrt = 0, mass_on = 0, open = 0;
+  // per group of 64 sites: 1 when a site of the group lies within 2 voxels of
+  // a non-periodic face (its ring-2 stencils leave the box)
+  std::vector<uint8_t> near_face;
   int par0 = 0;
   double M0 = 0.0;
   double params[32];
@@ -163,6 +166,16 @@ public:
       return FELBM_EUNSUPPORTED;
     }
     S.nblocks = (S.n + 255) / 256;
+    if (!(S.per[0] && S.per[1] && S.per[2])) {
+      const int dims[3] = { g->nx, g->ny, g->nz };
+      S.near_face.assign((size_t)((S.n + 63) / 64), 0);
+      for (int64_t i = 0; i < cnt; ++i) {
+        const uint32_t c = coords[(size_t)i];
+        const int v[3] = { (int)(c & ((1u << bx) - 1u)), (int)((c >> bx) & ((1u << by) - 1u)), (int)(c >> (bx + by)) };
+        for (int a = 0; a < 3; ++a)
+          if (!S.per[a] && (v[a] < 2 || v[a] > dims[a] - 3)) { S.near_face[(size_t)(i / 64)] = 1; break; }
+      }
+    }
     // one allocation: h, g0, g1 (19 ns), 9 fields + 4 export (ns), words, coords, acc
     const size_t RB = sizeof(real), ns = (size_t)S.ns;
     size_t off = 0;
@@ -914,12 +927,35 @@ private:
       descs.push_back(slots_[q].d);
       const int key = variant(slots_[q]);
       for (int b = 0; b < slots_[q].nblocks; ++b) all.push_back(make_int2(di, b * 256));
-      for (int bi = 0; bi < 3; ++bi)
-        for (int b = 0; b * kBS[bi] < slots_[q].n; ++b) var[bi][key].push_back(make_int2(di, b * kBS[bi]));
+      // open variant: a block with no site within 2 voxels of a non-periodic
+      // face, no open-boundary node and not site 0 is marked MAP_INTERIOR
+      // (its workgroup runs the periodic code)
+      std::vector<uint8_t> edge;
+      if (key & 4) {
+        const Slot<real> &S = slots_[q];
+        edge = S.near_face;
+        edge.resize((size_t)((S.n + 63) / 64), 0);
+        for (const ObEntry &e : S.entries) edge[(size_t)(e.site / 64)] = 1;
+        edge[0] = 1;
+      }
+      for (int bi = 0; bi < 3; ++bi) {
+        const int gpb = kBS[bi] / 64;
+        for (int b = 0; b * kBS[bi] < slots_[q].n; ++b) {
+          int x = di;
+          if (key & 4) {
+            bool near = false;
+            for (int gq = b * gpb; gq < (b + 1) * gpb && gq < (int)edge.size(); ++gq) near = near || edge[(size_t)gq];
+            if (!near) x |= MAP_INTERIOR;
+          }
+          var[bi][key].push_back(make_int2(x, b * kBS[bi]));
+        }
+      }
     }
     auto pad = [](std::vector<int2> &v) { while (v.size() % 8) v.push_back(make_int2(-1, 0)); };
     for (int bi = 0; bi < 3; ++bi)
       for (int k = 0; k < 8; ++k) {
+        // one instance, no interior marks: the kernel takes the descriptor
+        // index as an argument and reads no map
         int one = var[bi][k].empty() ? -1 : var[bi][k][0].x;
         for (const int2 &e : var[bi][k]) if (e.x != one) { one = -1; break; }
         vsdi_[bi][k] = one;

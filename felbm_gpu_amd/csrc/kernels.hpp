@@ -194,6 +194,18 @@ __device__ __forceinline__ int logical_block()
   return (int)((b & 7u) * (nb >> 3) + (b >> 3));      // nb is a multiple of 8
 }
 
+// A block-map entry (descriptor, first site) of an instance with non-periodic
+// faces or open-boundary nodes carries MAP_INTERIOR when none of its sites is
+// within 2 voxels of a non-periodic face or an open-boundary node (nor site
+// 0, whose block advances the step counters): the kernels of the open
+// variant run such a workgroup through the periodic code, the same
+// arithmetic without the in-box tests and boundary-node lookups.
+constexpr int MAP_INTERIOR = 1 << 30;
+__device__ __forceinline__ int2 map_entry(const int2 *__restrict__ map, int sdi)
+{
+  return sdi >= 0 ? make_int2(sdi, logical_block()*(int)blockDim.x) : map[logical_block()];
+}
+
 // ---- memory access ------------------------------------------------------------------
 // Generic pointers from the descriptor are cast to address space 1 for plain
 // global accesses; the SoA arrays go through raw buffer loads/stores so the
@@ -712,10 +724,8 @@ __device__ __forceinline__ void acc_vec(int m, real t, real &sx, real &sy, real 
 
 // ==== K1: moments (k_moments, device_engine.cuh:116-135, pass 1 only) ===========
 template <typename real, bool OPEN>
-__global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ D,
-                                                 const int2 *__restrict__ map, int sdi, unsigned T)
+__device__ __forceinline__ void moments_body(const Inst<real> *__restrict__ D, const int2 bm, unsigned T)
 {
-  const int2 bm = sdi >= 0 ? make_int2(sdi, logical_block()*(int)blockDim.x) : map[logical_block()];
   if (bm.x < 0) return;
   const Inst<real> &d = D[bm.x];
   const int i = bm.y + (int)threadIdx.x;
@@ -776,6 +786,17 @@ __global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ 
     unsigned long long v[5] = { (unsigned long long)m.hi, m.lo, (unsigned long long)w.hi, w.lo, m.bad + w.bad };
     block_accumulate<5>(v, d.acc + ACC_MASS + cur*5*NSLOT, NSLOT, (int)(blockIdx.x & (NSLOT - 1)));
   }
+}
+
+template <typename real, bool OPEN>
+__global__ void __launch_bounds__(256) k_moments(const Inst<real> *__restrict__ D,
+                                                 const int2 *__restrict__ map, int sdi, unsigned T)
+{
+  const int2 bm = map_entry(map, sdi);
+  if constexpr (OPEN) {
+    if (bm.x >= 0 && (bm.x & MAP_INTERIOR)) { moments_body<real, false>(D, make_int2(bm.x & ~MAP_INTERIOR, bm.y), T); return; }
+  }
+  moments_body<real, OPEN>(D, bm, T);
 }
 
 // ==== open boundaries: imposed fields (k_open_bnd_fields + k_open_bnd_buf_fields,
@@ -927,11 +948,8 @@ __device__ __forceinline__ real lap_term(bool hin, bool hip, real fin, real fip,
 // (device_engine.cuh:149-196).  exp != nullptr also writes rho, ux, uy, uz for
 // a field download (multiphase_gpu.cuh:704-757).
 template <typename real, bool HBB, bool OPEN>
-__global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D,
-                                                const int2 *__restrict__ map, int sdi, unsigned T,
-                                                real *exp, int exp_inst)
+__device__ __forceinline__ void cfield_body(const Inst<real> *__restrict__ D, const int2 bm, unsigned T, real *exp, int exp_inst)
 {
-  const int2 bm = sdi >= 0 ? make_int2(sdi, logical_block()*(int)blockDim.x) : map[logical_block()];
   if (bm.x < 0) return;
   const Inst<real> &d = D[bm.x];
   const unsigned cur = (T ^ (unsigned)d.par0) & 1u;
@@ -1045,12 +1063,22 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
   }
 }
 
+template <typename real, bool HBB, bool OPEN>
+__global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D,
+                                                const int2 *__restrict__ map, int sdi, unsigned T,
+                                                real *exp, int exp_inst)
+{
+  const int2 bm = map_entry(map, sdi);
+  if constexpr (OPEN) {
+    if (bm.x >= 0 && (bm.x & MAP_INTERIOR)) { cfield_body<real, HBB, false>(D, make_int2(bm.x & ~MAP_INTERIOR, bm.y), T, exp, exp_inst); return; }
+  }
+  cfield_body<real, HBB, OPEN>(D, bm, T, exp, exp_inst);
+}
+
 // ==== K3: lap(mu) (k_pack_lap_zero + k_lap_mf) ======================================
 template <typename real, bool HBB, bool OPEN>
-__global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ D,
-                                                 const int2 *__restrict__ map, int sdi, unsigned T)
+__device__ __forceinline__ void mufield_body(const Inst<real> *__restrict__ D, const int2 bm, unsigned T)
 {
-  const int2 bm = sdi >= 0 ? make_int2(sdi, logical_block()*(int)blockDim.x) : map[logical_block()];
   if (bm.x < 0) return;
   const Inst<real> &d = D[bm.x];
   if (bm.y >= d.n) return;                          // a padding workgroup: no sites
@@ -1095,6 +1123,17 @@ __global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ 
   }
 }
 
+template <typename real, bool HBB, bool OPEN>
+__global__ void __launch_bounds__(256) k_mufield(const Inst<real> *__restrict__ D,
+                                                 const int2 *__restrict__ map, int sdi, unsigned T)
+{
+  const int2 bm = map_entry(map, sdi);
+  if constexpr (OPEN) {
+    if (bm.x >= 0 && (bm.x & MAP_INTERIOR)) { mufield_body<real, HBB, false>(D, make_int2(bm.x & ~MAP_INTERIOR, bm.y), T); return; }
+  }
+  mufield_body<real, HBB, OPEN>(D, bm, T);
+}
+
 // Store one post-collision population k of the site at byte offset io into
 // its own slot k (contiguous across the wave); the next gather reads it from
 // x + e_k, or -- neighbour not a site -- from slot opp(k) of x as the
@@ -1120,10 +1159,8 @@ __device__ __forceinline__ void store_post(real v, rsrc_t r, int k, unsigned vo,
 // (:547-651), with the field quantities of the reference's separate kernels
 // (k_moments u, k_grad_cd/bd/md, k_vel_press_corr) recomputed in registers.
 template <typename real, bool MRT, bool HBB, bool OPEN>
-__global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ D,
-                                                 const int2 *__restrict__ map, int sdi, unsigned T, int stats)
+__device__ __forceinline__ void collide_body(const Inst<real> *__restrict__ D, const int2 bm, unsigned T, int stats)
 {
-  const int2 bm = sdi >= 0 ? make_int2(sdi, logical_block()*(int)blockDim.x) : map[logical_block()];
   if (bm.x < 0) return;
   const Inst<real> &d = D[bm.x];
   const unsigned cur = (T ^ (unsigned)d.par0) & 1u;
@@ -1377,6 +1414,17 @@ __global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ 
                                  (unsigned long long)s4.hi, s4.lo, s0.bad + s3.bad + s4.bad, vmax };
     block_accumulate<12>(v, d.acc + ACC_STATS, NSLOT, (int)(blockIdx.x & (NSLOT - 1)), true);
   }
+}
+
+template <typename real, bool MRT, bool HBB, bool OPEN>
+__global__ void __launch_bounds__(256) k_collide(const Inst<real> *__restrict__ D,
+                                                 const int2 *__restrict__ map, int sdi, unsigned T, int stats)
+{
+  const int2 bm = map_entry(map, sdi);
+  if constexpr (OPEN) {
+    if (bm.x >= 0 && (bm.x & MAP_INTERIOR)) { collide_body<real, MRT, HBB, false>(D, make_int2(bm.x & ~MAP_INTERIOR, bm.y), T, stats); return; }
+  }
+  collide_body<real, MRT, HBB, OPEN>(D, bm, T, stats);
 }
 
 }  // namespace fm
