@@ -295,13 +295,38 @@ public:
     return FELBM_OK;
   }
 
+  // The rest state h_k = c w_k, g_k = w_k p0 (the reference's initial
+  // condition), built on the device: only c crosses the bus (8 B per site
+  // instead of the 304 B of a double state), staged in the field block, which
+  // holds nothing between steps that the next step reads.  Same bits as
+  // upload_state of the host-built state (k_rest_stage).
   int init_rest(int slot, const double *c, double p0) override {
     if (int e = check(slot)) return e;
     Slot<real> &S = slots_[slot];
     const size_t n = (size_t)S.n;
-    std::vector<double> h(19 * n), g(19 * n);
-    for (int k = 0; k < 19; ++k) for (size_t i = 0; i < n; ++i) { h[k * n + i] = c[i] * WD[k]; g[k * n + i] = WD[k] * p0; }
-    return upload_state(slot, h.data(), g.data());
+    std::vector<double> ce;
+    const double *src = c;
+    if (!S.perm.empty()) {
+      ce.resize(n);
+      for (size_t i = 0; i < n; ++i) ce[eng(S, i)] = c[i];
+      src = ce.data();
+    }
+    double *cd = (double *)S.d.craw;            // 13 ns reals >= n doubles
+    const unsigned cur = cur_of(S);
+    real *stage = cur ? S.d.g0 : S.d.g1;        // the dead g buffer between steps
+    const int nb = (S.n + 255) / 256;
+    FM_HIP(hipMemcpyAsync(cd, src, n * sizeof(double), hipMemcpyHostToDevice, stream_));
+    for (int pass = 0; pass < 2; ++pass) {
+      hipLaunchKernelGGL(k_rest_stage<real>, dim3(nb), dim3(256), 0, stream_, S.d, (const double *)cd, p0, pass, stage);
+      FM_LAUNCH();
+      real *dst = pass ? (cur ? S.d.g1 : S.d.g0) : S.d.h;
+      hipLaunchKernelGGL(k_unstream<real>, dim3(nb), dim3(256), 0, stream_, S.d, (const real *)stage, dst);
+      FM_LAUNCH();
+    }
+    FM_HIP(hipMemsetAsync(S.d.acc + ACC_FLAGS, 0, sizeof(unsigned long long), stream_));
+    FM_HIP(hipStreamSynchronize(stream_));
+    S.stats_valid = 0;
+    return FELBM_OK;
   }
 
   int download_state(int slot, double *h, double *g) override {

@@ -1540,6 +1540,29 @@ __global__ void __launch_bounds__(256) k_unstream(Inst<real> d, const real *in, 
   }
 }
 
+// The rest state of felbm_init_rest in the post-stream layout (population k
+// of site i at k*ns + i; k_unstream then stores it for the next gather):
+// h_k = c_i w_k (which = 0) or g_k = w_k p0 (which = 1), formed in double and
+// rounded once to the state's type (FP32 g as its deviation from w_k, flushed
+// below 2^-31) -- the operations and roundings of a host-built state passed
+// through upload_state, so the two give the same bits.  c is in engine order.
+template <typename real>
+__global__ void __launch_bounds__(256) k_rest_stage(Inst<real> d, const double *c, double p0, int which, real *out)
+{
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const size_t ns = (size_t)d.ns;
+  const double ci = c[i];
+  #pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    real v;
+    if (which == 0) v = (real)__dmul_rn(ci, WD[k]);
+    else if constexpr (sizeof(real) == 4) v = g_flush((float)__dsub_rn(__dmul_rn(WD[k], p0), (double)wf32(k)*p_ref<real>()));
+    else v = (real)__dmul_rn(WD[k], p0);
+    out[(size_t)k*ns + (size_t)i] = v;
+  }
+}
+
 // reference-format stencil/streaming tables (multiphase_gpu.cuh:93-208)
 // regenerated from the engine's neighbour logic, for bitwise comparison
 // iperm (may be null): the caller's number of each engine site; tables are

@@ -125,3 +125,38 @@ def test_site_list_is_validated():
     g = MultiPhaseGPU("f64").init(solid, prm, {}, site_xyz=xyz)      # the engine's own order
     assert g.n == len(xyz)
     g.batch.free()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("permuted", [False, True])
+def test_init_rest_equals_uploaded_rest_state(precision, permuted):
+    """init_rest builds the rest state h_k = c w_k, g_k = w_k p0 on the device
+    from c alone; it must give the same bits as that state built on the host
+    and passed through upload_state, before and after steps, in the engine's
+    own numbering and a permuted one, with an off-unity rest pressure."""
+    solid = W.spheres(20, 18, 16, 2.5, 4.0, 0.5, 33)
+    prm = W.derive_params(theta_deg=70.0, acceleration=(1e-5, 0.0, 0.0))
+    sw = {"use_mrt": 1, "correct_op_mass": 1}
+    xyz = _xyz(solid)
+    n = len(xyz)
+    P = _perm(n, 9) if permuted else np.arange(n)
+    kw = {"site_xyz": xyz[P]} if permuted else {}
+    wk = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    c = W.slab_c(solid, 0, 0, 10, 4.0)[P]
+    p0 = 1.0 + 3.7e-4
+    a = MultiPhaseGPU(precision).init(solid, prm, sw, **kw)
+    b = MultiPhaseGPU(precision).init(solid, prm, sw, **kw)
+    a.init_rest(c, p0)
+    b.upload_state((wk[:, None] * c[None, :]).ravel(), (wk[:, None] * np.full(n, p0)[None, :]).ravel())
+    a.record_target_mass()
+    b.record_target_mass()
+    assert a.get_target_mass() == b.get_target_mass()
+    _say("init_rest %s permuted=%s: %d sites" % (precision, permuted, n))
+    for steps in (0, 12):
+        a.step(steps)
+        b.step(steps)
+        ha, ga = a.download_state()
+        hb, gb = b.download_state()
+        assert np.array_equal(ha, hb) and np.array_equal(ga, gb), steps
+    a.free()
+    b.free()
