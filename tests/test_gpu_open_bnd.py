@@ -33,9 +33,9 @@ def channel(copy, seed=3, shape=(10, 24, 12)):
     return W.open_channel(solid, 2, copy_to_buffers=copy, split_axis=0)
 
 
-def make_open_pair(copy, ob, use_mrt, precision="f64", c0=0.0, seed=3):
+def make_open_pair(copy, ob, use_mrt, precision="f64", c0=0.0, seed=3, shape=(10, 24, 12)):
     from oracle.pyoracle import Oracle
-    s, inl, out, ic, bv, br = channel(copy, seed)
+    s, inl, out, ic, bv, br = channel(copy, seed, shape)
     prm = W.derive_params(theta_deg=60.0, acceleration=(0.0, 1e-6, 0.0))
     o = Oracle(s, prm, use_mrt=use_mrt, periodic=PER, open_bnd=ob)
     o.set_open_bnd(inl, out, ic, bv, br, copy)
@@ -82,6 +82,39 @@ def test_buffered_open_boundaries_alternate_injection_bitexact(use_mrt):
     o, g = make_open_pair(True, BUFFERED, use_mrt)
     o.run(40)
     g.step(40)
+    compare(o, g)
+
+
+def _interior_fraction(s, inl, out, bv):
+    """Share of 256-site blocks the engine runs through the periodic code:
+    no site within 2 voxels of the open (y) faces, no boundary node, not the
+    block of site 0 (the rule of the engine's block maps)."""
+    x, y, z = W.site_coords(s)
+    n, ny = len(y), s.shape[1]
+    near = (y < 2) | (y > ny - 3)
+    near[np.asarray(list(inl) + list(out) + list(bv), dtype=np.int64)] = True
+    g = np.zeros((n + 63) // 64, bool)
+    np.logical_or.at(g, np.arange(n) // 64, near)
+    g[0] = True
+    blocks = [not g[b * 4:(b + 1) * 4].any() for b in range((n + 255) // 256)]
+    return float(np.mean(blocks))
+
+
+@pytest.mark.parametrize("copy", [False, True])
+def test_interior_blocks_bitexact(copy):
+    """A channel wide enough that most blocks hold no site near the open faces
+    and no boundary node: the engine runs those through the periodic code
+    (one launch per pass, the open code for the rest), bit-identical to the
+    oracle."""
+    shape = (8, 40, 48)
+    ob = BUFFERED if copy else DIRECT
+    s, inl, out, ic, bv, br = channel(copy, 5, shape)
+    frac = _interior_fraction(s, inl, out, bv)
+    _say("interior blocks, copy_to_buffers", copy, "interior share %.2f" % frac)
+    assert frac > 0.5
+    o, g = make_open_pair(copy, ob, 1, seed=5, shape=shape)
+    o.run(30)
+    g.step(30)
     compare(o, g)
 
 
