@@ -14,8 +14,9 @@ serve     a stream of solve requests with Poisson arrivals.  Each request is an
           until the mean order parameter changes by less than `tol` (relative)
           over one check interval or `max_steps` are done.  The server admits
           arrived requests while the instances fit the HBM budget, advances all
-          active instances together `check_every` steps at a time, and retires
-          converged ones.  Latency = completion - arrival; aggregate MLUPS =
+          active instances together `admit_every` steps at a time, checks each
+          request every `check_every` of its own steps, and retires converged
+          ones.  Latency = completion - arrival; aggregate MLUPS =
           sum_i n_i steps_i / makespan.
 
 The scheduler (Server) talks to an engine through four calls -- admit, step,
@@ -248,18 +249,24 @@ class GpuBackend:
 class Server:
     """Continuous batching with HBM admission control.
 
-    Each tick: admit every arrived request that fits the budget (FIFO), advance
-    all active instances by `check_every` steps, then retire those whose mean c
-    moved by less than `tol` relative to the previous check, or that reached
-    `max_steps`.  `clock` returns seconds since the stream started."""
+    Each tick: admit every arrived request that fits the budget (FIFO) and
+    advance all active instances by `admit_every` steps.  A request is checked
+    at every multiple of `check_every` of its own steps: it is retired when its
+    mean c moved by less than `tol` relative to its previous check, or when it
+    reached `max_steps`.  The tick sets how long an arrival waits for a running
+    batch (at most one tick) independently of the convergence window.
+    `clock` returns seconds since the stream started."""
 
     def __init__(self, backend, hbm_budget=200e9, check_every=500, tol=1e-6, max_steps=20000, precision="f32",
-                 clock=None, sleep=None):
+                 clock=None, sleep=None, admit_every=100):
         self.be = backend
         self.budget = float(hbm_budget)
         self.check_every = int(check_every)
+        self.tick = int(admit_every) if admit_every else self.check_every
         self.tol = float(tol)
         self.max_steps = int(max_steps)
+        if self.tick <= 0 or self.check_every % self.tick or self.max_steps % self.tick:
+            raise ValueError("check_every and max_steps must be multiples of admit_every")
         self.precision = precision
         self.clock = clock
         self.sleep = sleep or time.sleep
@@ -290,11 +297,14 @@ class Server:
                 if pending:
                     self.sleep(max(0.0, pending[0]["arrival"] - clock()))
                 continue
-            self.be.step(self.check_every)
-            lu += sum(a["n"] for a in active) * float(self.check_every)
+            self.be.step(self.tick)
+            lu += sum(a["n"] for a in active) * float(self.tick)
             still = []
             for a in active:
-                a["steps"] += self.check_every
+                a["steps"] += self.tick
+                if a["steps"] % self.check_every and a["steps"] < self.max_steps:
+                    still.append(a)
+                    continue
                 try:
                     m = self.be.mean_c(a["h"])
                 except FelbmError as err:
@@ -347,7 +357,7 @@ def _served_fields(problems, precision, steps):
 
 
 def run_serve(count, rate=4.0, seed=2024, precision="f32", check_every=500, tol=1e-6, max_steps=20000,
-              hbm_budget=200e9, log=None, check_fp64_steps=0):
+              hbm_budget=200e9, log=None, check_fp64_steps=0, admit_every=100):
     """The served stream; check_fp64_steps > 0 also runs every request problem
     that many steps in FP32 and in FP64 and reports the largest FP32 - FP64
     difference of c and u over them."""
@@ -358,7 +368,8 @@ def run_serve(count, rate=4.0, seed=2024, precision="f32", check_every=500, tol=
         log("serve: %d requests generated in %.1fs" % (len(reqs), time.time() - t_gen))
     be = GpuBackend(precision)
     try:
-        res = Server(be, hbm_budget, check_every, tol, max_steps, precision).run(reqs, problems, log)
+        res = Server(be, hbm_budget, check_every, tol, max_steps, precision,
+                     admit_every=admit_every).run(reqs, problems, log)
     finally:
         be.close()
     res["rate_per_s"] = rate

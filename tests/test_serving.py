@@ -20,8 +20,9 @@ class FakeClock:
 
 
 class FakeBackend:
-    """mean c of request i approaches 1 as 1 - r_i^k after k checks; one step
-    of one site costs 1e-9 s of simulated time."""
+    """mean c of request i approaches 1 as 1 - r_i^k after k check intervals
+    (500 steps) of its own; one step of one site costs 1e-9 s of simulated
+    time."""
 
     def __init__(self, clock, rates, sizes):
         self.clock, self.rates, self.sizes = clock, rates, sizes
@@ -37,10 +38,10 @@ class FakeBackend:
     def step(self, nsteps):
         self.clock.t += 1e-9 * nsteps * sum(self.sizes[r] for r in self.active)
         for r in self.active:
-            self.active[r] += 1
+            self.active[r] += nsteps
 
     def mean_c(self, rid):
-        return 1.0 - self.rates[rid] ** self.active[rid]
+        return 1.0 - self.rates[rid] ** (self.active[rid] // 500)
 
     def retire(self, rid):
         del self.active[rid]
@@ -118,7 +119,7 @@ class BlowUpBackend(FakeBackend):
         self.bad = bad
 
     def mean_c(self, rid):
-        if rid == self.bad and self.active[rid] >= 3:
+        if rid == self.bad and self.active[rid] >= 3 * 500:
             raise S.FelbmError(S.FELBM_ENONFINITE, "non-finite or out-of-range value in the monitored fields")
         return super().mean_c(rid)
 
@@ -140,6 +141,27 @@ def test_nonfinite_request_is_retired_and_frees_its_budget():
     admit_t = {rid: t for ev, rid, t in be.log if ev == "admit"}
     retire_t = {rid: t for ev, rid, t in be.log if ev == "retire"}
     assert admit_t[2] == retire_t[1]
+
+
+def test_arrivals_join_a_running_batch_within_one_tick():
+    """Admission is checked every `admit_every` steps, not only at the
+    convergence checks: a request arriving while others run waits at most one
+    tick (100 steps of the running batch), and every request is still checked
+    at multiples of 500 of its own steps."""
+    n = 8
+    sizes = {i: 200000 for i in range(n)}
+    clk = FakeClock()
+    be = FakeBackend(clk, {i: 0.999999 for i in range(n)}, sizes)
+    reqs, probs = make(n, sizes)
+    res = S.Server(be, hbm_budget=1e12, check_every=500, tol=1e-6, max_steps=3000, clock=clk,
+                   sleep=clk.sleep, admit_every=100).run(reqs, probs)
+    admit_t = {rid: t for ev, rid, t in be.log if ev == "admit"}
+    for r in reqs:
+        # one tick of the largest batch: 100 steps x every request's sites
+        assert admit_t[r["id"]] - r["arrival"] <= 1e-9 * 100 * sum(sizes.values()) + 1e-12
+    assert all(d["steps"] == 3000 and d["status"] == "step cap" for d in res["done"])
+    with np.testing.assert_raises(ValueError):
+        S.Server(be, check_every=500, admit_every=300)
 
 
 def test_open_boundary_allocation_is_in_the_budget():
