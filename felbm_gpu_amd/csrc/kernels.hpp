@@ -1056,9 +1056,12 @@ __device__ __forceinline__ void cfield_body(const Inst<real> *__restrict__ D, co
     if (ts % (unsigned long long)d.tr_vskip == 0) {
       const long ns = d.ns;
       // one record per site, [ux, uy, uz, c], in one vector store (c is read
-      // only for the phase split of the stretching rows)
+      // only for the phase split of the stretching rows).  Non-temporal: the
+      // tracer kernel reads the records of a few cells per tracer, and the
+      // full snapshot (16 B per site, FP32) would otherwise displace the mu
+      // stencil data of the next pass from L2.
       typedef real v4 __attribute__((ext_vector_type(4)));
-      ((__attribute__((address_space(1))) v4 *)d.tu)[i] = v4{ux, uy, uz, c};
+      __builtin_nontemporal_store(v4{ux, uy, uz, c}, (__attribute__((address_space(1))) v4 *)d.tu + i);
     }
   }
 }
