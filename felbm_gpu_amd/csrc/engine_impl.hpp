@@ -952,26 +952,26 @@ private:
       descs.push_back(slots_[q].d);
       const int key = variant(slots_[q]);
       for (int b = 0; b < slots_[q].nblocks; ++b) all.push_back(make_int2(di, b * 256));
-      // open variant: a block with no site within 2 voxels of a non-periodic
-      // face, no open-boundary node and not site 0 is marked MAP_INTERIOR
-      // (its workgroup runs the periodic code)
-      std::vector<uint8_t> edge;
+      // open variant: a 256-site group with no site within 2 voxels of a
+      // non-periodic face, no open-boundary node and not site 0 is interior,
+      // and every block of it, at any workgroup shape, is marked MAP_INTERIOR
+      // (its workgroup runs the periodic code).  Deciding per 256-site group
+      // keeps each site on the same code at every shape: in FP32 the two
+      // bodies may round a term differently where FMA contraction differs.
+      std::vector<uint8_t> edge, inner;
       if (key & 4) {
         const Slot<real> &S = slots_[q];
         edge = S.near_face;
         edge.resize((size_t)((S.n + 63) / 64), 0);
         for (const ObEntry &e : S.entries) edge[(size_t)(e.site / 64)] = 1;
         edge[0] = 1;
+        inner.assign((size_t)((S.n + 255) / 256), 1);
+        for (size_t gq = 0; gq < edge.size(); ++gq) if (edge[gq]) inner[gq / 4] = 0;
       }
       for (int bi = 0; bi < 3; ++bi) {
-        const int gpb = kBS[bi] / 64;
         for (int b = 0; b * kBS[bi] < slots_[q].n; ++b) {
           int x = di;
-          if (key & 4) {
-            bool near = false;
-            for (int gq = b * gpb; gq < (b + 1) * gpb && gq < (int)edge.size(); ++gq) near = near || edge[(size_t)gq];
-            if (!near) x |= MAP_INTERIOR;
-          }
+          if ((key & 4) && inner[(size_t)(b * kBS[bi] / 256)]) x |= MAP_INTERIOR;
           var[bi][key].push_back(make_int2(x, b * kBS[bi]));
         }
       }
