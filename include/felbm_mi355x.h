@@ -97,7 +97,11 @@ int  felbm_num_instances(felbm_engine *e, int *count);
 /* ---- state ---------------------------------------------------------------- */
 int  felbm_upload_state(felbm_engine *e, int slot, const double *h, const double *g);   /* :354 */
 int  felbm_download_state(felbm_engine *e, int slot, double *h, double *g);            /* :449 */
-/* rest equilibrium h_k = w_k c, g_k = w_k p0 from a per-site c (n values) */
+/* rest equilibrium h_k = w_k c, g_k = w_k p0 from a per-site c (n values),
+   built on the device.  No MultiPhaseGPU member: the reference driver builds
+   the initial populations on the host and passes them to upload_state
+   (felbm_gpu_main.cu:322); for a rest initial condition this call gives the
+   same state bit for bit with 8 bytes per site over the bus. */
 int  felbm_init_rest(felbm_engine *e, int slot, const double *c_site, double p0);
 
 /* ---- stepping -------------------------------------------------------------
