@@ -84,15 +84,19 @@ typedef struct {
 /* ---- lifecycle -------------------------------------------------------------
  * precision: 8 = FP64 distributions (reference double build),
  *            4 = FP32 distributions with FP64 reductions (-DFELBM_SINGLE). */
+/* One engine per GPU holding any number of lattice instances (slots); the
+ * reference constructs one MultiPhaseGPU per lattice (felbm_gpu_main.cu:314). */
 int  felbm_create(felbm_engine **out, int device, int precision);
 int  felbm_destroy(felbm_engine *e);                         /* MultiPhaseGPU::free, :759 */
 /* MultiPhaseGPU::init (multiphase_gpu.cuh:75-351); returns the slot id. */
 int  felbm_add_instance(felbm_engine *e, const felbm_geometry *geom,
                         const felbm_params *prm, const felbm_switches *sw, int *slot);
+/* retire one instance of the engine (no counterpart: the reference frees a
+ * whole MultiPhaseGPU, free() :759) */
 int  felbm_remove_instance(felbm_engine *e, int slot);
 int  felbm_num_sites(felbm_engine *e, int slot, int64_t *n);  /* SubDomain::size_sites */
 int  felbm_state_size(felbm_engine *e, int slot, int64_t *vn);/* state_size, :457 */
-int  felbm_num_instances(felbm_engine *e, int *count);
+int  felbm_num_instances(felbm_engine *e, int *count);       /* no counterpart */
 
 /* ---- state ---------------------------------------------------------------- */
 int  felbm_upload_state(felbm_engine *e, int slot, const double *h, const double *g);   /* :354 */
@@ -115,8 +119,12 @@ int  felbm_set_forcing_factor(felbm_engine *e, int slot, double ff);
  * containing "sinusoidal" -> sin(2 pi t / period), "square" -> its sign,
  * anything else -> 1 */
 int  felbm_forcing_factor_at(const char *timedep, double period, int64_t t, double *ff);
+/* wait for the engine's stream (no counterpart: the reference's step()
+ * blocks on its per-step mass-correction copy, multiphase_gpu.cuh:636-639) */
 int  felbm_synchronize(felbm_engine *e);
-/* capture `nsteps` (even) steps in a HIP graph and replay it `replays` times */
+/* capture `nsteps` (even) steps in a HIP graph and replay it `replays` times
+ * (no counterpart: that per-step host copy keeps the reference's step out of
+ * graph capture) */
 int  felbm_step_graph(felbm_engine *e, int nsteps, int replays);
 
 /* ---- mass correction (record_target_mass / get/set / current_mass) -------- */
@@ -213,9 +221,11 @@ int  felbm_export_tables(felbm_engine *e, int slot, int32_t *cdm, int32_t *cdp,
 int  felbm_export_tables_noalloc(felbm_engine *e, int slot, int32_t *cdm, int32_t *cdp,
                                  int32_t *bcase, int32_t *bc1, int32_t *bc2, int32_t *avgnext,
                                  int32_t *lcase, int32_t *lc1, int32_t *lc2, int32_t *src);
-/* the 19x19 MRT moment matrix and its inverse used by the engine */
+/* the 19x19 MRT moment matrix and its inverse used by the engine (the
+ * reference's constant tables, device_engine.cuh:65-81) */
 int  felbm_mrt_matrices(double *M, double *Minv);
-/* last kernel timings of one step, in ms: moments, c-stencil, mu-stencil, collide */
+/* kernel timings of one step, in ms, averaged over nsteps: moments,
+ * c-stencil, mu-stencil, collide (measurement aid, no counterpart) */
 int  felbm_kernel_times(felbm_engine *e, int nsteps, float *ms4);
 
 /* Tile-shape autotuner for the batched engine (no reference counterpart: the
@@ -242,8 +252,10 @@ int  felbm_copy_probe(int device, int64_t bytes, int reps, double *gbs);
  * with.  No reference counterpart (measurement aid). */
 int  felbm_stream_probe(int device, int64_t bytes, int reps, int mode, double *gbs);
 
+/* message of the last failing call on this thread (the reference exits the
+ * process on any CUDA error, gpu_common.cuh:14-22) */
 const char *felbm_last_error(void);
-const char *felbm_version(void);
+const char *felbm_version(void);                             /* no counterpart */
 
 /* ---- geometry generators (DomainInitializer_* stand-ins) ------------------ */
 int     felbm_geom_cylinders(int nx, int ny, int nz, int count, double rmin, double rmax,
