@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=1000,
                     help="steps of the 48-instance batched sweep (0 skips it)")
     ap.add_argument("--serve", type=int, default=16, help="requests of the served stream (0 skips it)")
+    ap.add_argument("--serve-max-sites", type=float, default=6e6,
+                    help="lattice sites in flight the server admits up to (0: the HBM budget alone)")
     ap.add_argument("--err-steps", type=int, default=10000,
                     help="steps of the FP32-vs-FP64 field comparison on the same lattice, the 10,000 steps of "
                          "the 300^3 throughput case by default (0 skips it)")
@@ -219,10 +221,18 @@ def main():
     if args.serve > 0:
         eng.free()
         from felbm_gpu_amd import serving
-        res = serving.run_serve(args.serve, rate=4.0, seed=2024, precision="f32", log=log, check_fp64_steps=2000)
+        cap = int(args.serve_max_sites) or None
+        res = serving.run_serve(args.serve, rate=4.0, seed=2024, precision="f32", log=log, check_fp64_steps=2000,
+                                max_sites=cap)
         served = {k: v for k, v in res.items() if k != "done"}
         served["requests_steps"] = [d["steps"] for d in res["done"]]
         log("serve: %s" % json.dumps(served))
+        if cap is not None:
+            # the same stream admitted by the HBM budget alone, for comparison
+            alt = serving.run_serve(args.serve, rate=4.0, seed=2024, precision="f32", log=log)
+            served["hbm_budget_only"] = {k: alt[k] for k in ("p50_latency_s", "p99_latency_s", "aggregate_MLUPS",
+                                                             "makespan_s", "converged", "step_cap")}
+            log("serve, HBM budget alone: %s" % json.dumps(served["hbm_budget_only"]))
 
     cpu = None
     if not args.no_cpu:

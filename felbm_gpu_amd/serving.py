@@ -13,7 +13,8 @@ serve     a stream of solve requests with Poisson arrivals.  Each request is an
           buffer nodes at each end, copy_to_buffers, no mass correction), run
           until the mean order parameter changes by less than `tol` (relative)
           over one check interval or `max_steps` are done.  The server admits
-          arrived requests while the instances fit the HBM budget, advances all
+          arrived requests while the instances fit the HBM budget (and the
+          optional budget of lattice sites in flight), advances all
           active instances together `admit_every` steps at a time, checks each
           request every `check_every` of its own steps, and retires converged
           ones.  Latency = completion - arrival; aggregate MLUPS =
@@ -249,8 +250,15 @@ class GpuBackend:
 class Server:
     """Continuous batching with HBM admission control.
 
-    Each tick: admit every arrived request that fits the budget (FIFO) and
-    advance all active instances by `admit_every` steps.  A request is checked
+    Each tick: admit every arrived request that fits the budgets (FIFO) and
+    advance all active instances by `admit_every` steps.  Two budgets bound
+    what runs at once: device memory (`hbm_budget`, bytes) and, optionally,
+    compute (`max_sites`, lattice sites).  Every active instance advances at
+    the batch's step rate, which falls as the batch grows, so with memory as
+    the only limit every request of a burst shares the GPU until nearly the
+    end; a site budget large enough to keep the GPU full (a few million
+    sites) lets the first arrivals finish early at about the same aggregate
+    rate.  A request larger than either budget still runs, alone.  A request is checked
     at every multiple of `check_every` of its own steps: it is retired when its
     mean c moved by less than `tol` relative to its previous check, or when it
     reached `max_steps`.  The tick sets how long an arrival waits for a running
@@ -258,9 +266,10 @@ class Server:
     `clock` returns seconds since the stream started."""
 
     def __init__(self, backend, hbm_budget=200e9, check_every=500, tol=1e-6, max_steps=20000, precision="f32",
-                 clock=None, sleep=None, admit_every=100):
+                 clock=None, sleep=None, admit_every=100, max_sites=None):
         self.be = backend
         self.budget = float(hbm_budget)
+        self.max_sites = int(max_sites) if max_sites else None
         self.check_every = int(check_every)
         self.tick = int(admit_every) if admit_every else self.check_every
         self.tol = float(tol)
@@ -277,22 +286,26 @@ class Server:
         pending = sorted(requests, key=lambda r: r["arrival"])
         active, done = [], []
         used = 0.0
+        sites = 0
         lu = 0.0
         while pending or active:
             now = clock()
-            # admission (FIFO, budget permitting)
+            # admission (FIFO, budgets permitting)
             while pending and pending[0]["arrival"] <= now:
                 r = pending[0]
                 pr = problems[r["id"]]
                 n_open = sum(len(pr.get(k, ())) for k in ("inlet", "outlet", "buf_verts"))
-                need = instance_bytes(int(np.count_nonzero(pr["solid"] == 0)), self.precision, n_open)
-                if used + need > self.budget and active:
+                n_sites = int(np.count_nonzero(pr["solid"] == 0))
+                need = instance_bytes(n_sites, self.precision, n_open)
+                if active and (used + need > self.budget or
+                               (self.max_sites is not None and sites + n_sites > self.max_sites)):
                     break
                 pending.pop(0)
                 h, n = self.be.admit(problems[r["id"]])
                 active.append({"req": r, "h": h, "n": n, "steps": 0, "last": None, "bytes": need,
-                               "admitted": clock()})
+                               "sites": n_sites, "admitted": clock()})
                 used += need
+                sites += n_sites
             if not active:
                 if pending:
                     self.sleep(max(0.0, pending[0]["arrival"] - clock()))
@@ -319,6 +332,7 @@ class Server:
                     t = clock()
                     self.be.retire(a["h"])
                     used -= a["bytes"]
+                    sites -= a["sites"]
                     status = "converged" if conv else ("nonfinite" if not finite else "step cap")
                     done.append({"id": a["req"]["id"], "arrival": a["req"]["arrival"], "done": t,
                                  "latency": t - a["req"]["arrival"], "steps": a["steps"], "sites": a["n"],
@@ -341,7 +355,7 @@ class Server:
                 "step_cap": int(sum(d["status"] == "step cap" for d in done)),
                 "rel_change_at_retirement": {"min": min(rels), "median": float(np.median(rels)), "max": max(rels)}
                 if rels else None,
-                "lattice_updates": lu, "done": done}
+                "lattice_updates": lu, "hbm_budget": self.budget, "max_sites": self.max_sites, "done": done}
 
 
 def _served_fields(problems, precision, steps):
@@ -357,7 +371,7 @@ def _served_fields(problems, precision, steps):
 
 
 def run_serve(count, rate=4.0, seed=2024, precision="f32", check_every=500, tol=1e-6, max_steps=20000,
-              hbm_budget=200e9, log=None, check_fp64_steps=0, admit_every=100):
+              hbm_budget=200e9, log=None, check_fp64_steps=0, admit_every=100, max_sites=None):
     """The served stream; check_fp64_steps > 0 also runs every request problem
     that many steps in FP32 and in FP64 and reports the largest FP32 - FP64
     difference of c and u over them."""
@@ -369,7 +383,7 @@ def run_serve(count, rate=4.0, seed=2024, precision="f32", check_every=500, tol=
     be = GpuBackend(precision)
     try:
         res = Server(be, hbm_budget, check_every, tol, max_steps, precision,
-                     admit_every=admit_every).run(reqs, problems, log)
+                     admit_every=admit_every, max_sites=max_sites).run(reqs, problems, log)
     finally:
         be.close()
     res["rate_per_s"] = rate

@@ -167,3 +167,37 @@ def test_arrivals_join_a_running_batch_within_one_tick():
 def test_open_boundary_allocation_is_in_the_budget():
     assert S.instance_bytes(10 ** 6, "f32", n_open=20000) - S.instance_bytes(10 ** 6, "f32") == \
         4 * 10 ** 6 + 8 * 4 * 20000 + 512
+
+
+def test_site_budget_bounds_the_batch_and_lets_early_arrivals_finish_first():
+    """With a budget of sites in flight the batch never holds more than the
+    budget (a request larger than it runs alone), admission stays FIFO, and
+    on a burst of equal requests the median latency falls against memory-only
+    admission while the last request finishes no later than the step rate
+    allows."""
+    n = 12
+    sizes = {i: 300000 if i != 5 else 900000 for i in range(n)}
+    runs = {}
+    for cap in (None, 800000):
+        clk = FakeClock()
+        be = FakeBackend(clk, {i: 0.999999 for i in range(n)}, sizes)
+        reqs, probs = make(n, sizes)
+        res = S.Server(be, hbm_budget=1e12, check_every=500, tol=1e-6, max_steps=2000, clock=clk,
+                       sleep=clk.sleep, max_sites=cap).run(reqs, probs)
+        live, peak, order, alone = set(), 0, [], True
+        for ev, rid, t in be.log:
+            if ev == "admit":
+                live.add(rid)
+                order.append(rid)
+                if rid == 5:
+                    alone = live == {5}
+            else:
+                live.discard(rid)
+            if cap is not None and 5 not in live:
+                peak = max(peak, sum(sizes[r] for r in live))
+        assert order == sorted(order) and res["requests"] == n
+        assert res["max_sites"] == cap
+        if cap is not None:
+            assert peak <= cap and alone
+        runs[cap] = res
+    assert runs[800000]["p50_latency_s"] < runs[None]["p50_latency_s"]
