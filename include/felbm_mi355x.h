@@ -88,7 +88,10 @@ typedef struct {
  * reference constructs one MultiPhaseGPU per lattice (felbm_gpu_main.cu:314). */
 int  felbm_create(felbm_engine **out, int device, int precision);
 int  felbm_destroy(felbm_engine *e);                         /* MultiPhaseGPU::free, :759 */
-/* MultiPhaseGPU::init (multiphase_gpu.cuh:75-351); returns the slot id. */
+/* MultiPhaseGPU::init (multiphase_gpu.cuh:75-351); returns the slot id.
+ * One instance holds fewer than 2^32 bytes per distribution array
+ * (19 * sites * sizeof(real)): at most about 56.5M FP32 or 28M FP64 sites;
+ * a larger one returns FELBM_EUNSUPPORTED. */
 int  felbm_add_instance(felbm_engine *e, const felbm_geometry *geom,
                         const felbm_params *prm, const felbm_switches *sw, int *slot);
 /* retire one instance of the engine (no counterpart: the reference frees a
