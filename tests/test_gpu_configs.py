@@ -225,3 +225,23 @@ def test_3d_image_stack_ingested_runs_bitexact(tmp_path):
     fo, fg = o.download(), g.download()
     for k in fo:
         assert np.array_equal(fo[k], fg[k]), k
+
+
+@pytest.mark.parametrize("precision,edge", [("f32", 390), ("f64", 310)])
+def test_instance_beyond_32bit_offsets_is_refused(precision, edge):
+    """An all-fluid box whose distribution array would reach 2^32 bytes
+    (19 * sites * sizeof(real)) is refused before anything is allocated, and
+    the engine keeps working for the next instance."""
+    from felbm_gpu_amd import FelbmError
+    prm = W.derive_params()
+    sw = {"use_mrt": 1, "correct_op_mass": 0}
+    b = Batch(precision=precision)
+    with pytest.raises(FelbmError) as err:
+        MultiPhaseGPU(batch=b).init(np.zeros((edge, edge, edge), np.uint8), prm, sw)
+    assert err.value.code == -5 and "4 GiB" in str(err.value)
+    _say("refused %d^3 %s: %s" % (edge, precision, err.value))
+    small = MultiPhaseGPU(batch=b).init(W.spheres(16, 16, 16, 2.0, 3.0, 0.6, 9), prm, sw)
+    small.init_rest(np.full(small.n, 0.3), 1.0)
+    b.step(2)
+    assert np.isfinite(small.download()["c"]).all()
+    b.free()
