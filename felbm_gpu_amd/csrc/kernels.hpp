@@ -408,7 +408,7 @@ __device__ __forceinline__ RowWin row_win(rsrc_t rW, unsigned row, unsigned wb)
 // RB) of its site, or of the centre site itself when it is not a site, and sets
 // fluid / solid-inside-the-box flags.  Branch-free.
 template <typename real, bool OPEN>
-__device__ __forceinline__ unsigned nb_lookup(const Inst<real> &d, rsrc_t rW, const Nbhd &nb,
+__device__ __forceinline__ unsigned nb_lookup(const Inst<real> & /*d*/, rsrc_t rW, const Nbhd &nb,
                                               int dx, int dy, int dz, unsigned self_off,
                                               bool &fluid, bool &solid)
 {
@@ -1054,7 +1054,6 @@ __device__ __forceinline__ void cfield_body(const Inst<real> *__restrict__ D, co
   if (!exp && d.trk) {          // tracers read this step's u (felbm_gpu_main.cu:752-763)
     const unsigned long long ts = ldg(d.acc, ACC_FLAGS + 2);
     if (ts % (unsigned long long)d.tr_vskip == 0) {
-      const long ns = d.ns;
       // one record per site, [ux, uy, uz, c], in one vector store (c is read
       // only for the phase split of the stretching rows).  Non-temporal: the
       // tracer kernel reads the records of a few cells per tracer, and the
@@ -1080,7 +1079,7 @@ __global__ void __launch_bounds__(256) k_cfield(const Inst<real> *__restrict__ D
 
 // ==== K3: lap(mu) (k_pack_lap_zero + k_lap_mf) ======================================
 template <typename real, bool HBB, bool OPEN>
-__device__ __forceinline__ void mufield_body(const Inst<real> *__restrict__ D, const int2 bm, unsigned T)
+__device__ __forceinline__ void mufield_body(const Inst<real> *__restrict__ D, const int2 bm, unsigned /*T*/)
 {
   if (bm.x < 0) return;
   const Inst<real> &d = D[bm.x];
